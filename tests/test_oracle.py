@@ -1,0 +1,296 @@
+"""Pins for the CPU oracle: closed forms, textbook expansions, brute force on
+tiny inputs and the reference's own scenarios (glibc rand()).  A plausible
+mistake in the oracle (a dropped Taylor term, a wrong ABM coefficient, an
+off-by-one in the scan seed or in an input sample index) fails one of these."""
+from fractions import Fraction
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import scenarios as sc
+from parode_amd import synth
+
+RNG = np.random.default_rng(1234)
+
+
+def rk4_step_f64(A, B, x, u0, uh, u1, h):
+    f = lambda x, u: A @ x + B @ u
+    k1 = f(x, u0)
+    k2 = f(x + h / 2 * k1, uh)
+    k3 = f(x + h / 2 * k2, uh)
+    k4 = f(x + h * k3, u1)
+    return x + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+# ---------------------------------------------------------------- RK4 map
+def test_rk4_diagonal_is_taylor_polynomial():
+    lam = np.array([-5.0, -0.1, -1.25, -20.0])
+    h = 1e-2
+    M, _ = orc.rk4_build_f64(np.diag(lam), np.zeros((4, 1)), h)
+    z = h * lam
+    taylor = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    np.testing.assert_allclose(np.diag(M), taylor, rtol=1e-15)
+    assert np.count_nonzero(M - np.diag(np.diag(M))) == 0
+    Mf, _ = orc.rk4_build_mirror(np.diag(lam).astype(np.float32), np.zeros((4, 1), np.float32), h)
+    np.testing.assert_array_equal(np.diag(Mf), taylor.astype(np.float32))
+
+
+def test_rk4_map_equals_classic_rk4_step():
+    n, m, h = 6, 3, 0.05
+    A = RNG.standard_normal((n, n)) - 3 * np.eye(n)
+    B = RNG.standard_normal((n, m))
+    M, Bb = orc.rk4_build_f64(A, B, h)
+    x = RNG.standard_normal(n)
+    u0, uh, u1 = (RNG.standard_normal(m) for _ in range(3))
+    ref = rk4_step_f64(A, B, x, u0, uh, u1, h)
+    got = M @ x + Bb @ np.concatenate([u0, uh, u1])
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_rk4_mirror_single_rounding_exact_inputs():
+    """Integer A, B and a power-of-two h make every float product exact, so the
+    mirror must equal the exact rational formula rounded once to float."""
+    n, m, h = 4, 2, 2.0 ** -4
+    A = RNG.integers(-2, 3, (n, n)).astype(np.float32)
+    B = RNG.integers(-2, 3, (n, m)).astype(np.float32)
+    M, Bb = orc.rk4_build_mirror(A, B, h)
+    Aq = [[Fraction(int(v)) for v in row] for row in A]
+    Bq = [[Fraction(int(v)) for v in row] for row in B]
+    H = Fraction(1, 16)
+
+    def mm(X, Y):
+        return [[sum(X[i][k] * Y[k][j] for k in range(len(Y))) for j in range(len(Y[0]))]
+                for i in range(len(X))]
+
+    A2 = mm(Aq, Aq); A3 = mm(A2, Aq); A4 = mm(Aq, A3)
+    AB = mm(Aq, Bq); A2B = mm(A2, Bq); A3B = mm(A3, Bq)
+    for i in range(n):
+        for j in range(n):
+            e = (1 if i == j else 0) + H * Aq[i][j] + H ** 2 / 2 * A2[i][j] \
+                + H ** 3 / 6 * A3[i][j] + H ** 4 / 24 * A4[i][j]
+            assert M[i, j] == np.float32(float(e))
+        for j in range(m):
+            b1 = (H * Bq[i][j] + H ** 2 * AB[i][j] + H ** 3 / 2 * A2B[i][j]
+                  + H ** 4 / 4 * A3B[i][j]) / 6
+            b2 = (4 * H * Bq[i][j] + 2 * H ** 2 * AB[i][j] + H ** 3 / 2 * A2B[i][j]) / 6
+            b3 = H * Bq[i][j] / 6
+            assert Bb[i, j] == np.float32(float(b1))
+            assert Bb[i, m + j] == np.float32(float(b2))
+            assert Bb[i, 2 * m + j] == np.float32(float(b3))
+
+
+# ---------------------------------------------------------------- ABM map
+def test_abm_companion_step_equals_pece_step():
+    n, m, h = 5, 2, 0.02
+    A = RNG.standard_normal((n, n)) - 2 * np.eye(n)
+    B = RNG.standard_normal((n, m))
+    M, Bb = orc.abm_build_f64(A, B, h)
+    xs = [RNG.standard_normal(n) for _ in range(4)]          # x_{k-3} .. x_k
+    us = [RNG.standard_normal(m) for _ in range(5)]          # u_{k-3} .. u_{k+1}
+    f = [A @ xs[i] + B @ us[i] for i in range(4)]            # f_{k-3} .. f_k
+    pred = xs[3] + h / 24 * (55 * f[3] - 59 * f[2] + 37 * f[1] - 9 * f[0])
+    fp = A @ pred + B @ us[4]
+    corr = xs[3] + h / 24 * (9 * fp + 19 * f[3] - 5 * f[2] + f[1])
+    X = np.concatenate(xs)
+    Xn = M @ X + np.concatenate([np.zeros(3 * n), Bb @ np.concatenate(us)])
+    np.testing.assert_allclose(Xn[:3 * n], X[n:], rtol=0, atol=0)
+    np.testing.assert_allclose(Xn[3 * n:], corr, rtol=1e-13, atol=1e-13)
+
+
+def test_abm_mirror_matches_f64_to_float_rounding():
+    n, m, h = 7, 3, 1e-3
+    A = (RNG.standard_normal((n, n)) - 2 * np.eye(n)).astype(np.float32)
+    B = RNG.standard_normal((n, m)).astype(np.float32)
+    Mf, Bf = orc.abm_build_mirror(A, B, h)
+    Md, Bd = orc.abm_build_f64(A.astype(np.float64), B.astype(np.float64), h)
+    np.testing.assert_allclose(Mf, Md, rtol=2e-6, atol=1e-9)
+    np.testing.assert_allclose(Bf, Bd, rtol=2e-6, atol=1e-12)
+
+
+# ------------------------------------------------------------ recurrences
+def test_recurrence_zero_input_is_matrix_power():
+    n, N = 6, 40
+    M = 0.3 * RNG.standard_normal((n, n)) + 0.5 * np.eye(n)
+    x0 = RNG.standard_normal(n)
+    X = orc.recurrence_f64(M, np.zeros((N, n)), x0)
+    for k in (0, 1, 7, 39):
+        np.testing.assert_allclose(X[k], np.linalg.matrix_power(M, k) @ x0, rtol=1e-12, atol=1e-12)
+
+
+def test_recurrence_constant_input_geometric_series():
+    n, N = 5, 64
+    M = 0.2 * RNG.standard_normal((n, n)) + 0.6 * np.eye(n)
+    v = RNG.standard_normal(n)
+    x0 = RNG.standard_normal(n)
+    X = orc.recurrence_f64(M, np.tile(v, (N, 1)), x0)
+    I = np.eye(n)
+    for k in (1, 13, 63):
+        Mk = np.linalg.matrix_power(M, k)
+        ref = Mk @ x0 + np.linalg.solve(I - M, (I - Mk) @ v)
+        np.testing.assert_allclose(X[k], ref, rtol=1e-11, atol=1e-11)
+    Xf = orc.recurrence_f32(M, np.tile(v, (N, 1)), x0)
+    assert orc.max_rel_err(Xf, X) < 1e-5
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 8, 17, 32, 64])
+def test_blelloch_mirror_equals_sequential(N):
+    """The reference's per-element up/down-sweep with the (I, x0) root seed
+    gives the exclusive scan, i.e. the sequential recurrence."""
+    na = 4
+    M = (0.25 * RNG.standard_normal((na, na)) + 0.7 * np.eye(na)).astype(np.float32)
+    Np2 = 1 << max(0, (N - 1).bit_length())
+    V = RNG.standard_normal((Np2, na)).astype(np.float32)
+    x0 = RNG.standard_normal(na).astype(np.float32)
+    Xs, tot = orc.blelloch_mirror_f32(M, V, x0, cw=4)
+    Xr = orc.recurrence_f64(M.astype(np.float64), V.astype(np.float64), x0.astype(np.float64))
+    assert orc.max_rel_err(Xs[:N], Xr[:N]) < 1e-5
+    # the carried accumulator is the state after all Np2 leaves
+    xfin = M.astype(np.float64) @ Xr[-1] + V[-1]
+    np.testing.assert_allclose(tot, xfin, rtol=1e-4, atol=1e-5)
+
+
+# -------------------------------------------------------------- inputs / grid
+def test_time_grid_mirrors_float_fma():
+    t = orc.time_grid_f32(0.1, 0.05 / 2, 1000)
+    tf, df = np.float32(0.1), np.float32(0.025)
+    ref = (tf.astype(np.float64) + df.astype(np.float64) * np.arange(1000)).astype(np.float32)
+    np.testing.assert_array_equal(t, ref)     # fma == exact product-sum rounded once
+
+
+def test_ufunc_family_f64_and_noise():
+    coef = np.array([[1, 0, 0, 0, 3.0, 0.5], [0, 1, 0.25, -0.5, 7.0, 1.0]])
+    U = orc.ufunc_eval_f64(coef, 0.0, 1e-3, 50)
+    t = np.arange(50) * 1e-3
+    np.testing.assert_allclose(U[:, 0], np.sin(3 * t + 0.5), rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(U[:, 1], np.cos(7 * t + 1.0) + 0.25 - 0.5 * t, rtol=1e-14, atol=1e-15)
+    g = synth.gauss_table(7, 20000, 3)
+    assert abs(g.mean()) < 0.02 and abs(g.std() - 1) < 0.02
+    for s, j in ((0, 0), (5, 2), (19999, 1)):
+        assert abs(orc.gauss_noise(7, s, j) - g[s, j]) < 1e-12
+
+
+def test_step_count_and_time_vector():
+    assert orc.step_count(0, 0.0, 4.9, 0.1) == int((4.9 - 0.0) / 0.1) + 1
+    assert orc.step_count(1, 0.0, 4.9, 0.1) == int((4.9 - 0.3) / 0.1) + 1 + 3
+    assert orc.step_count(1, 0.0, 0.1, 0.1) == -1
+    tv = orc.time_vector(0, 0.5, 0.1, 20)
+    acc = [np.float32(0.5)]
+    for _ in range(19):
+        acc.append(np.float32(np.float64(acc[-1]) + 0.1))
+    np.testing.assert_array_equal(tv, np.array(acc, np.float32))
+    tva = orc.time_vector(1, 0.0, 0.1, 6)
+    assert tva[0] == np.float32(0.3 - 0.3) and tva[3] == np.float32(0.0 + 0.3)
+
+
+# ------------------------------------------------------------ ABM start-up
+def test_abm_startup_is_three_classic_rk4_steps():
+    n, m, h = 4, 2, 0.05
+    A = (RNG.standard_normal((n, n)) - 2 * np.eye(n))
+    B = RNG.standard_normal((n, m))
+    x0 = RNG.standard_normal(n)
+    Uh = RNG.standard_normal((9, m))
+    X0 = orc.abm_startup_f64(A, B, x0, Uh, h)
+    x = x0.copy()
+    ref = [x0]
+    for i in range(3):
+        x = rk4_step_f64(A, B, x, Uh[2 * i], Uh[2 * i + 1], Uh[2 * i + 2], h)
+        ref.append(x)
+    np.testing.assert_allclose(X0, np.concatenate(ref), rtol=1e-14, atol=1e-14)
+    Xf = orc.abm_startup_f32(A.astype(np.float32), B.astype(np.float32), x0.astype(np.float32),
+                             Uh.astype(np.float32), h)
+    np.testing.assert_allclose(Xf, X0, rtol=1e-5, atol=1e-5)
+
+
+# ------------------------------------------------- whole solver, closed form
+def test_simulate_f64_diagonal_zero_input_closed_form():
+    lam = np.array([-1.0, -3.0, -0.5])
+    h, total = 0.01, 300
+    A = np.diag(lam)
+    y = orc.simulate_f64(0, A, None, None, None, np.ones(3), None, total, h)
+    z = h * lam
+    g = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    np.testing.assert_allclose(y, g[None, :] ** np.arange(total)[:, None], rtol=1e-12)
+
+
+def test_simulate_f64_abm_scalar_companion_closed_form():
+    """u = 0, scalar lambda: x_k follows the companion recurrence; compare with
+    the eigen-decomposition of the 4x4 companion matrix."""
+    lam, h, total = -2.0, 0.01, 200
+    M, _ = orc.abm_build_f64(np.array([[lam]]), np.zeros((1, 1)), h)
+    y, X = orc.simulate_f64(1, np.array([[lam]]), None, None, None, np.array([1.0]), None,
+                            total, h, want_states=True)
+    X0 = X[:4, 0]
+    w, P = np.linalg.eig(M)
+    c = np.linalg.solve(P, X0)
+    for k in (0, 10, 150):
+        Xk = (P @ (c * w ** k)).real
+        np.testing.assert_allclose(y[k + 3, 0], Xk[3], rtol=1e-10)
+    # and the start-up states are RK4 of x' = lam x
+    g = 1 + h * lam + (h * lam) ** 2 / 2 + (h * lam) ** 3 / 6 + (h * lam) ** 4 / 24
+    np.testing.assert_allclose(X0, g ** np.arange(4), rtol=1e-14)
+
+
+def test_mirror_and_f64_agree_on_config0():
+    s = synth.config(0, steps=512)
+    Uh64 = synth.samples_f64(s)
+    y64 = orc.simulate_f64(0, s.A.astype(np.float64), s.B.astype(np.float64),
+                           s.C.astype(np.float64), s.D.astype(np.float64),
+                           s.x0.astype(np.float64), Uh64, s.steps, s.h)
+    tv, ym = orc.simulate_mirror(0, s.A, s.B, s.C, s.D, s.x0, s.t0, s.t_end, s.h, ucoef=s.ucoef)
+    assert ym.shape[0] == s.steps
+    assert orc.max_rel_err(ym, y64) < 2e-5
+
+
+# ------------------------------------------ the reference's own scenarios
+def _l2(a, b):
+    return math.sqrt(float(np.sum((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
+
+
+@pytest.mark.parametrize("make", [sc.rk4_class, sc.abm_class])
+def test_reference_class_scenarios(make):
+    d = make()
+    t0, h, N = 0.0, d["h"], d["steps"]
+    t1 = (N - 1) * h + t0                                      # the tests' tEnd
+    tv, y = orc.simulate_mirror(d["solver"], d["A"], d["B"], d["C"], d["D"], d["x0"], t0, t1, h,
+                                ucoef=d["ucoef"])
+    total = y.shape[0]
+    Uh = sc.uhalf_from_double_sin(d["ucoef"], t0, h, total)
+    if d["solver"] == 0:
+        yc = orc.cpu_rk4_odeint(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, h)
+        assert _l2(y[1:], yc[1:]) < 0.1                         # TestRungeKutta4thOrder.cpp:142
+    else:
+        yc = orc.cpu_abm_pece(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, h)
+        assert _l2(y[4:total - 3], yc[4:total - 3]) < 0.1       # TestAdamsMoulton.cpp:227-230
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_reference_cpp_api_scenario(solver):
+    d = sc.cpp_api()
+    t0, h, N = 0.0, d["h"], d["steps"]
+    t1 = t0 + (N - 1) * h
+    tv, y = orc.simulate_mirror(solver, d["A"], d["B"], d["C"], d["D"], d["x0"], t0, t1, h,
+                                ucoef=d["ucoef"])
+    total = y.shape[0]
+    Uh = sc.uhalf_from_double_sin(d["ucoef"], t0, h, total)
+    cpu = orc.cpu_rk4_odeint if solver == 0 else orc.cpu_abm_pece
+    yc = cpu(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, h)
+    lo = 1 if solver == 0 else 4
+    assert _l2(y[lo:total - 3], yc[lo:total - 3]) < 0.1        # TestParODEInterface.cpp:104,259
+
+
+@pytest.mark.parametrize("steps", [100, 1000, 4000])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_reference_c_abi_scenario(solver, steps):
+    d = sc.c_abi(steps)
+    t0, h = 0.0, d["h"]
+    t1 = t0 + (steps - 1) * h
+    tv, y = orc.simulate_mirror(solver, d["A"], d["B"], d["C"], d["D"], d["x0"], t0, t1, h,
+                                ucoef=d["ucoef"])
+    total = y.shape[0]
+    Uh = sc.uhalf_from_double_sin(d["ucoef"], t0, h, total)
+    cpu = orc.cpu_rk4_odeint if solver == 0 else orc.cpu_abm_pece
+    yc = cpu(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, h)
+    lo = 1 if solver == 0 else 4
+    assert _l2(y[lo:total - 3], yc[lo:total - 3]) < 0.1        # TestParODEStaticLibCInterface.cpp:204,466
