@@ -1,0 +1,78 @@
+/*
+ * ParODEExt.h -- entry points beyond the reference's ten-function C ABI.
+ * None of these has a counterpart in the reference; each says which caller it
+ * is for.  All are C linkage, plain pointers and sizes; errors are the
+ * ErrorCode values of ParODEEnums.h (0 = OK).
+ */
+#ifndef PARODEEXT_H_
+#define PARODEEXT_H_
+
+#include <FTypeDef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision of the C ABI's SimulateODE: 0 = plain FP32 with the reference's
+ * rounding of M (default, parity mode), 1 = mixed (FP64 step map and powers,
+ * FP64 top scan levels, FP32 lower levels), 2 = FP64 scan. */
+void ParODE_SetPrecision(int mode);
+int ParODE_GetPrecision(void);
+
+/* Upper bound on the scan elements per batch (rounded down to a power of two);
+ * 0 restores the automatic plan from free device memory.  For callers that
+ * chain horizons explicitly and for tests of the batch carry. */
+void ParODE_SetMaxBatchSteps(long maxSteps);
+
+/* Compiles a UFunc source for a gfx target (e.g. "gfx950") without a GPU.
+ * Returns 0, or ParODE_CouldNotCreateGPUKernels with the compiler log. */
+int ParODE_CompileUFunc(const char* uFunc, const char* arch, char* log, int logLen);
+
+/* Text of the last exception caught at the ABI boundary. */
+void ParODE_GetLastErrorMessage(char* buf, int len);
+
+/* SimulateODE with sampled inputs instead of a registered UFunc: samples
+ * holds u(tStart + s*tStep/2), s = 0 .. 2*points, nInputs per row, as float
+ * (samplesDouble = 0) or double (1), on the host.  For inputs that are not a
+ * pure function of float time (noise, measured signals) and for the FP64
+ * precision path.  Other arguments as SimulateODE. */
+void SimulateODESampled(fType A[], fType B[], fType C[], fType D[],
+						int *nInputs, int *nStates, int *nOutputs,
+						int *ZeroB, int *ZeroC, int *ZeroD,
+						const void* samples, int *samplesDouble,
+						double *tStart, double *tEnd, double *tStep,
+						fType x0[], int *solver,
+						fType tVect[], fType xVect[], int *nSteps, int nErr[]);
+
+/* ---- device-resident plans: one system shape, inputs already in HBM ----
+ * For serving loops and benchmarks that keep matrices, samples and outputs on
+ * the GPU.  A plan owns its scan buffers; executing it launches the whole solve
+ * (step map, FP64 powers, leaves, sweeps, outputs) asynchronously on `stream`
+ * (a hipStream_t, or null for the default stream) with no host synchronisation.
+ * Device arrays: A n*n, B n*m, C p*n, D p*m, x0 n (float, row-major);
+ * samples (2*points+1)*m on the h/2 grid, float or double (fixed at creation);
+ * Y points*nOut (float, time-major; nOut = n when C and D are both absent). */
+int ParODE_PlanCreate(int solver, int nStates, int nInputs, int nOutputs,
+					  int zeroC, int zeroD, long points, double tStep,
+					  int precision, int samplesDouble, void** plan);
+int ParODE_PlanExecute(void* plan, const float* dA, const float* dB, const float* dC,
+					   const float* dD, const float* dx0, const void* dSamples,
+					   float* dY, void* stream);
+/* As ParODE_PlanExecute, then waits for the solve and writes the GPU time of
+ * its phases to phaseMs[6]: step map + FP64 powers, leaves (and ABM start-up),
+ * up-sweep, root, down-sweep, outputs (hipEvents on `stream`).  For
+ * benchmarks and profilers. */
+int ParODE_PlanExecuteTimed(void* plan, const float* dA, const float* dB, const float* dC,
+							const float* dD, const float* dx0, const void* dSamples,
+							float* dY, void* stream, float* phaseMs);
+long ParODE_PlanFootprint(void* plan);
+int ParODE_PlanLevels(void* plan);
+int ParODE_PlanDestroy(void* plan);
+/* Device memory the plan for this shape would hold, without allocating. */
+long ParODE_PlanEstimateBytes(int solver, int nStates, int nInputs, long points, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARODEEXT_H_ */

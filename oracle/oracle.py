@@ -50,6 +50,9 @@ def _load():
         "or_simulate_mirror": [_c_int, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p,
                                _c_int, _c_int, _opt_f32, _opt_f32, _c_double, _c_double,
                                _c_double, _f32p, _f32p, _f32p],
+        "or_simulate_mirror_exact": [_c_int, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p,
+                                     _c_int, _c_int, _opt_f32, _opt_f32, _c_double, _c_double,
+                                     _c_double, _f32p, _f64p],
         "or_simulate_f64": [_c_int, _c_int, _c_int, _c_int, _f64p, _f64p, _f64p, _f64p,
                             _c_int, _c_int, _f64p, _c_long, _c_double, _f64p, _f64p, _opt_f32],
         "or_cpu_rk4_odeint": [_c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p, _c_int, _c_int,
@@ -65,6 +68,7 @@ def _load():
     lib.or_step_count.restype = _c_long
     lib.or_simulate_mirror.restype = _c_long
     lib.or_simulate_f64.restype = _c_long
+    lib.or_simulate_mirror_exact.restype = _c_long
     lib.or_cpu_rk4_odeint.restype = _c_long
     lib.or_cpu_abm_pece.restype = _c_long
     lib.or_gauss_noise.restype = _c_double
@@ -253,6 +257,31 @@ def simulate_mirror(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None):
         t0, t1, h, _f32(x0), tv, xv)
     assert got == total
     return tv, xv
+
+
+def simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None):
+    """The reference's algorithm in exact arithmetic on the reference's float
+    inputs (FP32-mirror M, Bbar, samples and start-up states; FP64 scan).
+    Returns xVect[total, nOut] in FP64."""
+    A = _f32(A)
+    n = A.shape[0]
+    m = 0 if B is None or np.size(B) == 0 else np.shape(B)[1]
+    zC = int(C is None or np.size(C) == 0)
+    zD = int(D is None or np.size(D) == 0)
+    p = 0 if zC and zD else (np.shape(C)[0] if not zC else np.shape(D)[0])
+    nOut = n if (zC and zD) else p
+    total = step_count(solver, t0, t1, h)
+    if total < 1:
+        raise ValueError("empty horizon")
+    y = np.empty((total, nOut))
+    uc = _f32(ucoef) if ucoef is not None else None
+    uh = _f32(Uhalf) if Uhalf is not None else None
+    got = lib().or_simulate_mirror_exact(
+        solver, n, m, p, A, _nz(B, np.float32), _nz(C, np.float32), _nz(D, np.float32), zC, zD,
+        uc.ctypes.data if uc is not None else None, uh.ctypes.data if uh is not None else None,
+        t0, t1, h, _f32(x0), y)
+    assert got == total
+    return y
 
 
 def simulate_f64(solver, A, B, C, D, x0, Uhalf, total, h, want_states=False):

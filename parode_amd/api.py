@@ -1,0 +1,298 @@
+"""ctypes bindings of libParODEShared.so (the C ABI of ParODECInterface.h and
+the extensions of ParODEExt.h).
+
+The library is the product: every call here goes to the HIP implementation.
+A missing or unloadable library raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libParODEShared.so")
+
+# the ten C symbols of the reference's ParODECInterface.h:23-99
+C_ABI_SYMBOLS = ["GetErrorDescription", "InitializeAllGPUs", "GetAvailableGPUs", "GetDeviceName",
+                 "InitializeSelectedGPUs", "SimulateODE", "RegisterInput", "CloseGPUC",
+                 "StartTimer", "StopTimer"]
+EXT_SYMBOLS = ["ParODE_SetPrecision", "ParODE_GetPrecision", "ParODE_SetMaxBatchSteps",
+               "ParODE_CompileUFunc", "ParODE_GetLastErrorMessage", "SimulateODESampled",
+               "ParODE_PlanCreate", "ParODE_PlanExecute", "ParODE_PlanExecuteTimed",
+               "ParODE_PlanFootprint",
+               "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes"]
+
+RK4, ABM = 0, 1
+PLAIN32, MIXED, FP64 = 0, 1, 2
+
+ERRORS = {0: "ParODE_OK", -1: "ParODE_NoGPU", -2: "ParODE_NotInitialized",
+          -3: "ParODE_CouldNotCreateGPUKernels", -4: "ParODE_NotEnoughResources",
+          -5: "ParODE_InvalidDeviceId", -6: "ParODE_InvalidSystem",
+          -7: "ParODE_InvalidSolverType"}
+
+
+class ParODEError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)} ({code}) {last_error_message()}")
+
+
+_lib = None
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is not built; run `make -C parode_amd/csrc` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.GetErrorDescription.argtypes = [_ip, ctypes.c_char_p]
+        L.InitializeAllGPUs.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _ip]
+        L.GetAvailableGPUs.argtypes = [_ip, _ip, _ip]
+        L.GetDeviceName.argtypes = [_ip, ctypes.c_char_p, _ip]
+        L.InitializeSelectedGPUs.argtypes = [_ip, _ip, ctypes.c_char_p, ctypes.c_char_p, _ip]
+        L.SimulateODE.argtypes = [_fp, _fp, _fp, _fp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp,
+                                  _dp, _fp, _ip, _fp, _fp, _ip, _ip]
+        L.RegisterInput.argtypes = [ctypes.c_char_p, _ip, _ip]
+        L.CloseGPUC.argtypes = []
+        L.StartTimer.argtypes = [_ip]
+        L.StopTimer.argtypes = [_fp, _ip]
+        L.ParODE_SetPrecision.argtypes = [ctypes.c_int]
+        L.ParODE_GetPrecision.restype = ctypes.c_int
+        L.ParODE_SetMaxBatchSteps.argtypes = [ctypes.c_long]
+        L.ParODE_CompileUFunc.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.c_int]
+        L.ParODE_CompileUFunc.restype = ctypes.c_int
+        L.ParODE_GetLastErrorMessage.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.SimulateODESampled.argtypes = [_fp, _fp, _fp, _fp, _ip, _ip, _ip, _ip, _ip, _ip, _vp,
+                                         _ip, _dp, _dp, _dp, _fp, _ip, _fp, _fp, _ip, _ip]
+        L.ParODE_PlanCreate.argtypes = [ctypes.c_int] * 6 + [ctypes.c_long, ctypes.c_double,
+                                                             ctypes.c_int, ctypes.c_int,
+                                                             ctypes.POINTER(_vp)]
+        L.ParODE_PlanCreate.restype = ctypes.c_int
+        L.ParODE_PlanExecute.argtypes = [_vp] * 9
+        L.ParODE_PlanExecute.restype = ctypes.c_int
+        L.ParODE_PlanExecuteTimed.argtypes = [_vp] * 9 + [_fp]
+        L.ParODE_PlanExecuteTimed.restype = ctypes.c_int
+        L.ParODE_PlanFootprint.argtypes = [_vp]
+        L.ParODE_PlanFootprint.restype = ctypes.c_long
+        L.ParODE_PlanLevels.argtypes = [_vp]
+        L.ParODE_PlanLevels.restype = ctypes.c_int
+        L.ParODE_PlanDestroy.argtypes = [_vp]
+        L.ParODE_PlanDestroy.restype = ctypes.c_int
+        L.ParODE_PlanEstimateBytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_long, ctypes.c_int]
+        L.ParODE_PlanEstimateBytes.restype = ctypes.c_long
+        _lib = L
+    return _lib
+
+
+def last_error_message():
+    if _lib is None:
+        return ""
+    buf = ctypes.create_string_buffer(2048)
+    _lib.ParODE_GetLastErrorMessage(buf, 2048)
+    return buf.value.decode(errors="replace")
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_fp)
+
+
+# ------------------------------------------------------------------ C ABI
+def get_error_description(code):
+    buf = ctypes.create_string_buffer(512)
+    lib().GetErrorDescription(ctypes.byref(ctypes.c_int(code)), buf)
+    return buf.value.decode()
+
+
+def get_available_gpus():
+    n = ctypes.c_int(0)
+    ids = (ctypes.c_int * 64)()
+    e = ctypes.c_int(0)
+    lib().GetAvailableGPUs(ctypes.byref(n), ids, ctypes.byref(e))
+    return [ids[i] for i in range(n.value)], e.value
+
+
+def get_device_name(dev):
+    buf = ctypes.create_string_buffer(256)
+    e = ctypes.c_int(0)
+    lib().GetDeviceName(ctypes.byref(ctypes.c_int(dev)), buf, ctypes.byref(e))
+    return buf.value.decode(), e.value
+
+
+def initialize_all_gpus(kernel_dir=b"./", include_dir=b"./"):
+    e = ctypes.c_int(0)
+    lib().InitializeAllGPUs(kernel_dir, include_dir, ctypes.byref(e))
+    return e.value
+
+
+def initialize_selected_gpus(ids, kernel_dir=b"./", include_dir=b"./"):
+    arr = (ctypes.c_int * max(1, len(ids)))(*ids)
+    e = ctypes.c_int(0)
+    lib().InitializeSelectedGPUs(ctypes.byref(ctypes.c_int(len(ids))), arr, kernel_dir,
+                                 include_dir, ctypes.byref(e))
+    return e.value
+
+
+def register_input(src):
+    idx = ctypes.c_int(0)
+    e = ctypes.c_int(0)
+    lib().RegisterInput(src.encode() if isinstance(src, str) else src, ctypes.byref(idx),
+                        ctypes.byref(e))
+    return idx.value, e.value
+
+
+def close_gpu():
+    lib().CloseGPUC()
+
+
+def start_timer():
+    e = ctypes.c_int(0)
+    lib().StartTimer(ctypes.byref(e))
+    return e.value
+
+
+def stop_timer():
+    t = ctypes.c_float(0)
+    e = ctypes.c_int(0)
+    lib().StopTimer(ctypes.byref(t), ctypes.byref(e))
+    return t.value, e.value
+
+
+def set_precision(mode):
+    lib().ParODE_SetPrecision(mode)
+
+
+def set_max_batch_steps(n):
+    lib().ParODE_SetMaxBatchSteps(int(n))
+
+
+def compile_ufunc(src, arch="gfx950"):
+    log = ctypes.create_string_buffer(16384)
+    r = lib().ParODE_CompileUFunc(src.encode(), arch.encode(), log, 16384)
+    return r, log.value.decode(errors="replace")
+
+
+def _system_args(A, B, C, D, x0):
+    A = _f32(A)
+    n = A.shape[0]
+    zB = B is None or np.size(B) == 0
+    zC = C is None or np.size(C) == 0
+    zD = D is None or np.size(D) == 0
+    m = 0 if zB else np.shape(B)[1]
+    if zB and not zD:
+        m = np.shape(D)[1]
+    p = n if (zC and zD) else (np.shape(C)[0] if not zC else np.shape(D)[0])
+    Bf = None if zB else _f32(B)
+    Cf = None if zC else _f32(C)
+    Df = None if zD else _f32(D)
+    return A, Bf, Cf, Df, _f32(x0), n, m, p, int(zB), int(zC), int(zD)
+
+
+def simulate_ode(solver, A, B, C, D, u_index, t_start, t_end, t_step, x0, capacity=None):
+    """SimulateODE through the C ABI.  Returns (tVect, xVect[nSteps, nOut], nErr)."""
+    A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
+    nOut = n if (zC and zD) else p
+    cap = capacity or (int((t_end - t_start) / t_step) + 8)
+    tv = np.zeros(cap, np.float32)
+    xv = np.zeros((cap, max(nOut, n)), np.float32)
+    nsteps = ctypes.c_int(0)
+    e = ctypes.c_int(0)
+    lib().SimulateODE(_fptr(A), _fptr(Bf), _fptr(Cf), _fptr(Df), _i(m), _i(n), _i(p), _i(zB),
+                      _i(zC), _i(zD), _i(u_index), ctypes.byref(ctypes.c_double(t_start)),
+                      ctypes.byref(ctypes.c_double(t_end)), ctypes.byref(ctypes.c_double(t_step)),
+                      _fptr(x0f), _i(solver), _fptr(tv), _fptr(xv), ctypes.byref(nsteps),
+                      ctypes.byref(e))
+    k = nsteps.value
+    return tv[:k], xv.reshape(-1)[:k * nOut].reshape(k, nOut), e.value
+
+
+def simulate_ode_sampled(solver, A, B, C, D, samples, t_start, t_end, t_step, x0,
+                         capacity=None):
+    """SimulateODESampled: samples u(t_start + s h/2) (float32 or float64)."""
+    A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
+    nOut = n if (zC and zD) else p
+    cap = capacity or (int((t_end - t_start) / t_step) + 8)
+    tv = np.zeros(cap, np.float32)
+    xv = np.zeros((cap, max(nOut, n)), np.float32)
+    isd = samples is not None and np.asarray(samples).dtype == np.float64
+    S = None if samples is None else np.ascontiguousarray(samples,
+                                                          np.float64 if isd else np.float32)
+    nsteps = ctypes.c_int(0)
+    e = ctypes.c_int(0)
+    lib().SimulateODESampled(_fptr(A), _fptr(Bf), _fptr(Cf), _fptr(Df), _i(m), _i(n), _i(p),
+                             _i(zB), _i(zC), _i(zD),
+                             None if S is None else S.ctypes.data_as(_vp), _i(int(isd)),
+                             ctypes.byref(ctypes.c_double(t_start)),
+                             ctypes.byref(ctypes.c_double(t_end)),
+                             ctypes.byref(ctypes.c_double(t_step)), _fptr(x0f), _i(solver),
+                             _fptr(tv), _fptr(xv), ctypes.byref(nsteps), ctypes.byref(e))
+    k = nsteps.value
+    return tv[:k], xv.reshape(-1)[:k * nOut].reshape(k, nOut), e.value
+
+
+# ------------------------------------------------------------------- plans
+class Plan:
+    """A device-resident plan (ParODE_PlanCreate/Execute).  Arguments to
+    execute() are device pointers (ints), e.g. torch tensors' data_ptr()."""
+
+    def __init__(self, solver, n, m, p, zero_c, zero_d, points, h, precision, samples_double):
+        self.handle = ctypes.c_void_p()
+        r = lib().ParODE_PlanCreate(solver, n, m, p, int(zero_c), int(zero_d), points, h,
+                                    precision, int(samples_double), ctypes.byref(self.handle))
+        if r != 0:
+            raise ParODEError(r, "ParODE_PlanCreate")
+        self.solver, self.n, self.m, self.p, self.points = solver, n, m, p, points
+        self.n_out = n if (zero_c and zero_d) else p
+
+    def execute(self, dA, dB, dC, dD, dx0, dU, dY, stream=0):
+        r = lib().ParODE_PlanExecute(self.handle, dA, dB, dC, dD, dx0, dU, dY, stream)
+        if r != 0:
+            raise ParODEError(r, "ParODE_PlanExecute")
+
+    PHASES = ("map_powers", "leaves", "up_sweep", "root", "down_sweep", "outputs")
+
+    def execute_timed(self, dA, dB, dC, dD, dx0, dU, dY, stream=0):
+        """Execute, wait, and return {phase: GPU ms} measured with hipEvents."""
+        ms = (ctypes.c_float * 6)()
+        r = lib().ParODE_PlanExecuteTimed(self.handle, dA, dB, dC, dD, dx0, dU, dY, stream, ms)
+        if r != 0:
+            raise ParODEError(r, "ParODE_PlanExecuteTimed")
+        return dict(zip(self.PHASES, list(ms)))
+
+    @property
+    def footprint(self):
+        return lib().ParODE_PlanFootprint(self.handle)
+
+    @property
+    def levels(self):
+        return lib().ParODE_PlanLevels(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().ParODE_PlanDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

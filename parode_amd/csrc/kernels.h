@@ -1,0 +1,84 @@
+// Device kernels of the LTI scan and their host launchers (gfx950).
+//
+// Layouts (all row-major, device memory):
+//   trajectory X   : [Npad][na]  element-major; column e is the state x_e
+//   power table    : P_l = M^(2^l) in FP64, [L][na][na]
+//   level tables   : transposed copies PT_l[j][i] = P_l[i][j] (or P_l - I in
+//                    delta form), FP32 or FP64, [L][na][na]
+//   inputs U       : [rows][m] samples; element e reads rows e*sL + oL + k*rs
+//   outputs Y      : [N][nOut] time-major (the reference's xVect layout)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace parode {
+namespace k {
+
+// ---------------------------------------------------------------- step maps
+// Z = X * Y.  mirror = true: float products, sequential-k float accumulation,
+// no fused multiply-add (the rounding of uBLAS prod() on matrix<float>).
+void gemm_seq_f32(hipStream_t s, int r, int kk, int c, const float* X, const float* Y, float* Z);
+// FP64 tiled GEMM, Z = X * Y (used for A-products and the power squarings).
+void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z);
+void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out);
+void fill_f64(hipStream_t s, long count, double v, double* out);
+
+// RK4 combination (RungeKutta4thOrder.cpp:48-57).  mirror: inputs are the float
+// products, combination in double without contraction, one rounding to float;
+// M64/Bbar64 receive the float results promoted.  !mirror: FP64 products in,
+// FP64 results out.  Bbar32 may be null.
+void rk4_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
+                 const void* A3, const void* A4, const void* B, const void* AB, const void* A2B,
+                 const void* A3B, double* M64, double* Bbar64, float* Bbar32);
+// ABM companion (AdamsMoulton.cpp:254-316); M64 is 4n x 4n.
+void abm_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
+                 const void* B, const void* AB, double* M64, double* Bbar64, float* Bbar32);
+
+// Level tables from the FP64 powers: PT[l] = transpose(P_l - delta*I) as T.
+void make_level_table_f32(hipStream_t s, int na, const double* P, bool delta, float* PT);
+void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, double* PT);
+
+// ------------------------------------------------------------------- leaves
+// X[e][off + r] = sum_{q<k} Bbar[r][q*m + j] * U[(e*sL + oL + q*rs)*m + j], e < nReal;
+// rows off..off+n of X for e in [nReal, nPad) are zero; other rows of X are
+// zeroed when zeroOther (the ABM zero fill).  T: X type, TU: U type.
+template <typename T, typename TU, typename TB>
+void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, long nPad,
+            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X);
+
+// ------------------------------------------------------------ scan levels
+// Up-sweep level with stride d over np pairs: X[R] += P X[L] (R = (k+1)2d-1,
+// L = R-d); with delta tables X[R] += X[L] + (P-I) X[L].
+// Down-sweep: (X[L], X[R]) <- (X[R], P X[R] + X[L]).
+template <typename TX, typename TP>
+void level(hipStream_t s, bool down, bool delta, int na, long d, long np, const TP* PT, TX* X);
+
+// Root of the scan: tot = X[last]; X[last] = seed (seed as TS).
+template <typename TX, typename TS>
+void root(hipStream_t s, int na, long last, const TS* seed, TX* X, double* tot);
+// Carried accumulator for the next batch: seed' = P * seed + tot (FP64).
+void carry(hipStream_t s, int na, const double* P, const double* seed, const double* tot,
+           double* seedOut);
+
+// ------------------------------------------------------------------ outputs
+// Y[e][j] = sum_c C[j][c] X[e][oX*n + c] + sum_c D[j][c] U[(e*sO + oO)*m + c]
+// for e < N; with both absent Y[e] = X[e][oX*n ..].  Y stride nOut.
+template <typename TX, typename TU>
+void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
+             const float* C, const float* D, const TX* X, const TU* U, long sO, long oO,
+             float* Y);
+
+// ---------------------------------------------------------- ABM start-up
+// X0 = [x0; x1; x2; x3] by three classic RK4 steps on Uh (9 rows, h/2 grid).
+// mirror: float state, double stage combinations (odeint runge_kutta4
+// <vector<float>>), float A/B; else FP64 throughout.  Single work-group.
+template <typename TU>
+void abm_startup(hipStream_t s, bool mirror, int n, int m, double h, const float* A,
+                 const float* B, const float* x0, const TU* Uh, double* X0);
+
+// Copies a typed vector into FP64.
+template <typename T>
+void to_f64(hipStream_t s, long count, const T* in, double* out);
+
+}  // namespace k
+}  // namespace parode

@@ -1,0 +1,714 @@
+// gfx950 kernels of the parallel-in-time LTI scan.
+//
+// The scan is the reference's Blelloch up/down-sweep over the affine step maps
+// (Sources/GPUFixedStepIterative.cpp:236-398) after the LTI collapse: every
+// per-element matrix at level l equals P_l = M^(2^l), so a level is one
+// GEMM-shaped update  X[R] += P_l X[L]  (up) or  (X[L], X[R]) <- (X[R],
+// P_l X[R] + X[L])  (down) over the strided columns of the trajectory X, and
+// the reference's MM kernels, left-leaf and root copies collapse into the
+// power table and into the in-place pair update.
+#include "kernels.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace parode {
+namespace k {
+
+static void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("kernel launch failed: ") + what + ": " +
+                                 hipGetErrorString(e));
+}
+
+static inline unsigned blocks_for(long work, int block, long cap = 1L << 20)
+{
+    long b = (work + block - 1) / block;
+    if (b < 1) b = 1;
+    return (unsigned)std::min(b, cap);
+}
+
+// ------------------------------------------------------------------ GEMMs
+__global__ void k_gemm_seq_f32(int r, int kk, int c, const float* __restrict__ X,
+                               const float* __restrict__ Y, float* __restrict__ Z)
+{
+#pragma clang fp contract(off)
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)r * c) return;
+    int i = (int)(idx / c), j = (int)(idx % c);
+    float t = 0.0f;
+    for (int q = 0; q < kk; ++q) {
+        float pr = X[(long)i * kk + q] * Y[(long)q * c + j];
+        t = t + pr;
+    }
+    Z[idx] = t;
+}
+
+void gemm_seq_f32(hipStream_t s, int r, int kk, int c, const float* X, const float* Y, float* Z)
+{
+    long work = (long)r * c;
+    hipLaunchKernelGGL(k_gemm_seq_f32, dim3(blocks_for(work, 256)), dim3(256), 0, s, r, kk, c, X,
+                       Y, Z);
+    check_launch("gemm_seq_f32");
+}
+
+// 64x64 output tile, 16x16 threads with 4x4 micro-tiles, K-step 16, FP64 FMA.
+__global__ __launch_bounds__(256) void k_gemm_f64(int r, int kk, int c,
+                                                  const double* __restrict__ X,
+                                                  const double* __restrict__ Y,
+                                                  double* __restrict__ Z)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double As[BK][BM + 1];
+    __shared__ double Bs[BK][BN + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int q0 = 0; q0 < kk; q0 += BK) {
+        for (int t = threadIdx.x; t < BM * BK; t += 256) {
+            int i = t / BK, q = t % BK;
+            As[q][i] = (i0 + i < r && q0 + q < kk) ? X[(long)(i0 + i) * kk + q0 + q] : 0.0;
+        }
+        for (int t = threadIdx.x; t < BK * BN; t += 256) {
+            int q = t / BN, j = t % BN;
+            Bs[q][j] = (q0 + q < kk && j0 + j < c) ? Y[(long)(q0 + q) * c + j0 + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BK; ++q) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = As[q][ty + 16 * u];
+                b[u] = Bs[q][tx + 16 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int i = i0 + ty + 16 * u, j = j0 + tx + 16 * v;
+            if (i < r && j < c) Z[(long)i * c + j] = acc[u][v];
+        }
+}
+
+void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z)
+{
+    dim3 grid((c + 63) / 64, (r + 63) / 64);
+    hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    check_launch("gemm_f64");
+}
+
+__global__ void k_cast_f32_f64(long count, const float* __restrict__ in, double* __restrict__ out)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out)
+{
+    hipLaunchKernelGGL(k_cast_f32_f64, dim3(blocks_for(count, 256, 4096)), dim3(256), 0, s, count,
+                       in, out);
+    check_launch("cast_f32_to_f64");
+}
+
+// Zero/constant fill as a kernel (graph-capturable; no memset nodes).
+__global__ void k_fill_f64(long count, double v, double* __restrict__ out)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = v;
+}
+
+void fill_f64(hipStream_t s, long count, double v, double* out)
+{
+    hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(count, 256, 4096)), dim3(256), 0, s, count, v,
+                       out);
+    check_launch("fill_f64");
+}
+
+// ------------------------------------------------------------- step maps
+struct Rk4Coef {
+    double h, c2, c3, c4, s1, s2, s3, t0, t1, t2, u0;
+};
+
+// Scalar factors evaluated left to right as in RungeKutta4thOrder.cpp:48-57.
+static Rk4Coef rk4_coef(double h)
+{
+#pragma clang fp contract(off)
+    Rk4Coef c;
+    c.h = h;
+    c.c2 = 0.5 * h * h;
+    c.c3 = (1.0 / 6.0) * h * h * h;
+    c.c4 = (1.0 / 24.0) * h * h * h * h;
+    c.s1 = h * h;
+    c.s2 = 0.5 * h * h * h;
+    c.s3 = 0.25 * h * h * h * h;
+    c.t0 = 4.0 * h;
+    c.t1 = 2.0 * h * h;
+    c.t2 = 0.5 * h * h * h;
+    c.u0 = (1.0 / 6.0) * h;
+    return c;
+}
+
+template <typename TI>
+__global__ void k_rk4_combine(bool mirror, int n, int m, Rk4Coef k, const TI* __restrict__ A,
+                              const TI* __restrict__ A2, const TI* __restrict__ A3,
+                              const TI* __restrict__ A4, const TI* __restrict__ B,
+                              const TI* __restrict__ AB, const TI* __restrict__ A2B,
+                              const TI* __restrict__ A3B, double* __restrict__ MT64,
+                              double* __restrict__ Bbar64, float* __restrict__ Bbar32)
+{
+#pragma clang fp contract(off)
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nn = (long)n * n;
+    if (idx < nn) {
+        int i = (int)(idx / n), j = (int)(idx % n);
+        double v = ((i == j) ? 1.0 : 0.0) + k.h * (double)A[idx];
+        v = v + k.c2 * (double)A2[idx];
+        v = v + k.c3 * (double)A3[idx];
+        v = v + k.c4 * (double)A4[idx];
+        if (mirror) v = (double)(float)v;
+        MT64[(long)j * n + i] = v;                 // the power table is kept transposed
+        return;
+    }
+    idx -= nn;
+    if (idx >= (long)n * m) return;
+    int i = (int)(idx / m), j = (int)(idx % m);
+    double b = B[idx], ab = AB[idx], a2b = A2B[idx], a3b = A3B[idx];
+    double b1 = (1.0 / 6.0) * (((k.h * b + k.s1 * ab) + k.s2 * a2b) + k.s3 * a3b);
+    double b2 = (1.0 / 6.0) * ((k.t0 * b + k.t1 * ab) + k.t2 * a2b);
+    double b3 = k.u0 * b;
+    double vals[3] = {b1, b2, b3};
+    for (int q = 0; q < 3; ++q) {
+        long o = (long)i * 3 * m + q * m + j;
+        float f = (float)vals[q];
+        if (Bbar32) Bbar32[o] = f;
+        Bbar64[o] = mirror ? (double)f : vals[q];
+    }
+}
+
+void rk4_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
+                 const void* A3, const void* A4, const void* B, const void* AB, const void* A2B,
+                 const void* A3B, double* M64, double* Bbar64, float* Bbar32)
+{
+    Rk4Coef k = rk4_coef(h);
+    long work = (long)n * n + (long)n * m;
+    if (mirror)
+        hipLaunchKernelGGL(k_rk4_combine<float>, dim3(blocks_for(work, 256)), dim3(256), 0, s, true,
+                           n, m, k, (const float*)A, (const float*)A2, (const float*)A3,
+                           (const float*)A4, (const float*)B, (const float*)AB, (const float*)A2B,
+                           (const float*)A3B, M64, Bbar64, Bbar32);
+    else
+        hipLaunchKernelGGL(k_rk4_combine<double>, dim3(blocks_for(work, 256)), dim3(256), 0, s,
+                           false, n, m, k, (const double*)A, (const double*)A2, (const double*)A3,
+                           (const double*)A4, (const double*)B, (const double*)AB,
+                           (const double*)A2B, (const double*)A3B, M64, Bbar64, Bbar32);
+    check_launch("rk4_combine");
+}
+
+struct AbmCoef {
+    double a0A, a0A2, a1A, a1A2, a2A, a2A2, a3A2, bm1, b0B, b0AB, b1B, b1AB, b2B, b2AB, b3AB;
+};
+
+// AdamsMoulton.cpp:258-269, each factor evaluated left to right.
+static AbmCoef abm_coef(double h)
+{
+#pragma clang fp contract(off)
+    AbmCoef c;
+    c.a0A = 28.0 * h / 24.0;
+    c.a0A2 = 495.0 * h * h / 576.0;
+    c.a1A = -5.0 * h / 24.0;
+    c.a1A2 = -531.0 * h * h / 576.0;
+    c.a2A = 1.0 * h / 24.0;
+    c.a2A2 = 333.0 * h * h / 576.0;
+    c.a3A2 = -81.0 * h * h / 576.0;
+    c.bm1 = 9.0 * h / 24.0;
+    c.b0B = 19.0 * h / 24.0;
+    c.b0AB = 495.0 * h * h / 576.0;
+    c.b1B = -5.0 * h / 24.0;
+    c.b1AB = -531.0 * h * h / 576.0;
+    c.b2B = h / 24.0;
+    c.b2AB = 333.0 * h * h / 576.0;
+    c.b3AB = -81.0 * h * h / 576.0;
+    return c;
+}
+
+template <typename TI>
+__global__ void k_abm_combine(bool mirror, int n, int m, AbmCoef c, const TI* __restrict__ A,
+                              const TI* __restrict__ A2, const TI* __restrict__ B,
+                              const TI* __restrict__ AB, double* __restrict__ MT64,
+                              double* __restrict__ Bbar64, float* __restrict__ Bbar32)
+{
+#pragma clang fp contract(off)
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nn = (long)n * n;
+    const long na = 4L * n;
+    auto rnd = [mirror](double v) { return mirror ? (double)(float)v : v; };
+    if (idx < nn) {
+        int r = (int)(idx / n), q = (int)(idx % n);
+        double id = (r == q) ? 1.0 : 0.0, a = A[idx], a2 = A2[idx];
+        double blk[4] = {rnd(c.a3A2 * a2), rnd(c.a2A * a + c.a2A2 * a2),
+                         rnd(c.a1A * a + c.a1A2 * a2), rnd((id + c.a0A * a) + c.a0A2 * a2)};
+        const long row = 3L * n + r;
+        for (int b = 0; b < 4; ++b) MT64[(long)(b * n + q) * na + row] = blk[b];   // transposed
+        if (r == q) {
+            MT64[(long)(n + r) * na + r] = 1.0;              // M[r][n+r]
+            MT64[(long)(2 * n + r) * na + n + r] = 1.0;      // M[n+r][2n+r]
+            MT64[(long)(3 * n + r) * na + 2 * n + r] = 1.0;  // M[2n+r][3n+r]
+        }
+        return;
+    }
+    idx -= nn;
+    if (idx >= (long)n * m) return;
+    int r = (int)(idx / m), q = (int)(idx % m);
+    double b = B[idx], ab = AB[idx];
+    double vals[5] = {c.b3AB * ab, c.b2B * b + c.b2AB * ab, c.b1B * b + c.b1AB * ab,
+                      c.b0B * b + c.b0AB * ab, c.bm1 * b};
+    for (int k = 0; k < 5; ++k) {
+        long o = (long)r * 5 * m + (long)k * m + q;
+        float f = (float)vals[k];
+        if (Bbar32) Bbar32[o] = f;
+        Bbar64[o] = mirror ? (double)f : vals[k];
+    }
+}
+
+void abm_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
+                 const void* B, const void* AB, double* M64, double* Bbar64, float* Bbar32)
+{
+    AbmCoef c = abm_coef(h);
+    long na = 4L * n;
+    fill_f64(s, na * na, 0.0, M64);
+    long work = (long)n * n + (long)n * m;
+    if (mirror)
+        hipLaunchKernelGGL(k_abm_combine<float>, dim3(blocks_for(work, 256)), dim3(256), 0, s, true,
+                           n, m, c, (const float*)A, (const float*)A2, (const float*)B,
+                           (const float*)AB, M64, Bbar64, Bbar32);
+    else
+        hipLaunchKernelGGL(k_abm_combine<double>, dim3(blocks_for(work, 256)), dim3(256), 0, s,
+                           false, n, m, c, (const double*)A, (const double*)A2, (const double*)B,
+                           (const double*)AB, M64, Bbar64, Bbar32);
+    check_launch("abm_combine");
+}
+
+// -------------------------------------------------------------- level tables
+template <typename T>
+__global__ void k_level_table(long nn, int na, const double* __restrict__ PT64, bool delta,
+                              T* __restrict__ PT)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+         i += (long)gridDim.x * blockDim.x) {
+        double v = PT64[i];
+        if (delta && (i / na) == (i % na)) v -= 1.0;
+        PT[i] = (T)v;
+    }
+}
+
+void make_level_table_f32(hipStream_t s, int na, const double* P, bool delta, float* PT)
+{
+    long nn = (long)na * na;
+    hipLaunchKernelGGL(k_level_table<float>, dim3(blocks_for(nn, 256, 8192)), dim3(256), 0, s, nn,
+                       na, P, delta, PT);
+    check_launch("level_table_f32");
+}
+
+void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, double* PT)
+{
+    long nn = (long)na * na;
+    hipLaunchKernelGGL(k_level_table<double>, dim3(blocks_for(nn, 256, 8192)), dim3(256), 0, s, nn,
+                       na, P, delta, PT);
+    check_launch("level_table_f64");
+}
+
+// ------------------------------------------------------------------ leaves
+template <typename T>
+struct Acc {
+    using type = T;
+};
+
+template <typename T, typename TU, typename TB>
+__global__ void k_leaves(int n, int m, int kq, int na, int off, long nReal, long nPad,
+                         const TB* __restrict__ Bbar, const TU* __restrict__ U, long sL, long oL,
+                         long rs, T* __restrict__ X)
+{
+#pragma clang fp contract(off)
+    using TA = decltype(TB(0) * TU(0));
+    const long total = nPad * na;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long e = idx / na;
+        int rho = (int)(idx % na);
+        TA v = 0;
+        int r = rho - off;
+        if (e < nReal && r >= 0 && r < n && m > 0) {
+            const TB* brow = Bbar + (long)r * kq * m;
+            for (int q = 0; q < kq; ++q) {
+                const TU* u = U + (e * sL + oL + (long)q * rs) * m;
+                for (int j = 0; j < m; ++j) {
+                    TA pr = (TA)brow[q * m + j] * (TA)u[j];
+                    v = v + pr;
+                }
+            }
+        }
+        X[idx] = (T)v;
+    }
+}
+
+template <typename T, typename TU, typename TB>
+void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, long nPad,
+            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X)
+{
+    long work = nPad * na;
+    hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3(blocks_for(work, 256, 65536)), dim3(256), 0, s,
+                       n, m, kq, na, off, nReal, nPad, Bbar, U, sL, oL, rs, X);
+    check_launch("leaves");
+}
+
+template void leaves<float, float, float>(hipStream_t, int, int, int, int, int, long, long,
+                                          const float*, const float*, long, long, long, float*);
+template void leaves<float, double, double>(hipStream_t, int, int, int, int, int, long, long,
+                                            const double*, const double*, long, long, long,
+                                            float*);
+template void leaves<double, double, double>(hipStream_t, int, int, int, int, int, long, long,
+                                             const double*, const double*, long, long, long,
+                                             double*);
+template void leaves<double, float, double>(hipStream_t, int, int, int, int, int, long, long,
+                                            const double*, const float*, long, long, long,
+                                            double*);
+
+// ------------------------------------------------------------ scan levels
+// One work-group owns TC whole column pairs (all na rows), so the down-sweep
+// can update X[L] and X[R] in place.  The B operand (X[L] up, X[R] down) is
+// staged in LDS with a padded pitch; the transposed level table is read
+// row-coalesced (thread i reads PT[j][i]).  Thread t computes row
+// i = t mod rowThreads (+ multiples of rowThreads) for the columns
+// slot, slot + slots, ... (CPT of them).
+constexpr int kLevelThreads = 256;
+
+template <typename TX, typename TP, bool DOWN, bool DELTA, int kCPT>
+__global__ __launch_bounds__(kLevelThreads) void k_level(int na, long d, long np, int slots,
+                                                         int rowThreads, const TP* __restrict__ PT,
+                                                         TX* __restrict__ X)
+{
+    extern __shared__ unsigned char smem_raw[];
+    TP* Bs = reinterpret_cast<TP*>(smem_raw);
+    const int TC = slots * kCPT, pitch = TC + 1;
+    const long k0 = (long)blockIdx.x * TC;
+    const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
+    for (int t = threadIdx.x; t < na * TC; t += kLevelThreads) {
+        int c = t / na, i = t % na;
+        TP v = 0;
+        if (c < ncol) {
+            long R = (k0 + c + 1) * 2 * d - 1;
+            long src = DOWN ? R : R - d;
+            v = (TP)X[src * na + i];
+        }
+        Bs[i * pitch + c] = v;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int row0 = t % rowThreads, slot = t / rowThreads;
+    if (slot >= slots) return;
+    for (int i = row0; i < na; i += rowThreads) {
+        TP acc[kCPT];
+#pragma unroll
+        for (int q = 0; q < kCPT; ++q) acc[q] = 0;
+        const TP* bcol = Bs + slot;
+        for (int j = 0; j < na; ++j) {
+            const TP p = PT[(long)j * na + i];
+            const TP* brow = bcol + j * pitch;
+#pragma unroll
+            for (int q = 0; q < kCPT; ++q) acc[q] = fma(p, brow[q * slots], acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kCPT; ++q) {
+            int c = slot + q * slots;
+            if (c >= ncol) continue;
+            long R = (k0 + c + 1) * 2 * d - 1, L = R - d;
+            TP b = Bs[i * pitch + c];
+            TP y = DELTA ? acc[q] + b : acc[q];
+            if (!DOWN) {
+                X[R * na + i] = (TX)((TP)X[R * na + i] + y);
+            } else {
+                TP xl = (TP)X[L * na + i];
+                X[L * na + i] = (TX)b;
+                X[R * na + i] = (TX)(y + xl);
+            }
+        }
+    }
+}
+
+template <typename TX, typename TP, int CPT>
+static void level_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
+                         int slots, int rowThreads, const TP* PT, TX* X)
+{
+    int TC = slots * CPT;
+    size_t lds = (size_t)na * (TC + 1) * sizeof(TP);
+    unsigned grid = (unsigned)((np + TC - 1) / TC);
+    dim3 g(grid), b(kLevelThreads);
+    if (down) {
+        if (delta)
+            hipLaunchKernelGGL((k_level<TX, TP, true, true, CPT>), g, b, lds, s, na, d, np, slots,
+                               rowThreads, PT, X);
+        else
+            hipLaunchKernelGGL((k_level<TX, TP, true, false, CPT>), g, b, lds, s, na, d, np, slots,
+                               rowThreads, PT, X);
+    } else {
+        if (delta)
+            hipLaunchKernelGGL((k_level<TX, TP, false, true, CPT>), g, b, lds, s, na, d, np, slots,
+                               rowThreads, PT, X);
+        else
+            hipLaunchKernelGGL((k_level<TX, TP, false, false, CPT>), g, b, lds, s, na, d, np,
+                               slots, rowThreads, PT, X);
+    }
+    check_launch("level");
+}
+
+template <typename TX, typename TP>
+void level(hipStream_t s, bool down, bool delta, int na, long d, long np, const TP* PT, TX* X)
+{
+    if (np <= 0) return;
+    int rowThreads = na >= kLevelThreads ? kLevelThreads : na;
+    int slots = std::max(1, kLevelThreads / rowThreads);
+    // largest columns-per-thread whose LDS stage fits in 64 KiB
+    auto fits = [&](int cpt) { return (size_t)na * (slots * cpt + 1) * sizeof(TP) <= 65536; };
+    if (fits(16))
+        level_launch<TX, TP, 16>(s, down, delta, na, d, np, slots, rowThreads, PT, X);
+    else if (fits(8))
+        level_launch<TX, TP, 8>(s, down, delta, na, d, np, slots, rowThreads, PT, X);
+    else if (fits(4))
+        level_launch<TX, TP, 4>(s, down, delta, na, d, np, slots, rowThreads, PT, X);
+    else
+        level_launch<TX, TP, 2>(s, down, delta, na, d, np, slots, rowThreads, PT, X);
+}
+
+template void level<float, float>(hipStream_t, bool, bool, int, long, long, const float*, float*);
+template void level<float, double>(hipStream_t, bool, bool, int, long, long, const double*,
+                                   float*);
+template void level<double, double>(hipStream_t, bool, bool, int, long, long, const double*,
+                                    double*);
+
+// -------------------------------------------------------------------- root
+template <typename TX, typename TS>
+__global__ void k_root(int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,
+                       double* __restrict__ tot)
+{
+    for (int i = threadIdx.x; i < na; i += blockDim.x) {
+        if (tot) tot[i] = (double)X[last * na + i];
+        X[last * na + i] = (TX)seed[i];
+    }
+}
+
+template <typename TX, typename TS>
+void root(hipStream_t s, int na, long last, const TS* seed, TX* X, double* tot)
+{
+    hipLaunchKernelGGL((k_root<TX, TS>), dim3(1), dim3(256), 0, s, na, last, seed, X, tot);
+    check_launch("root");
+}
+
+template void root<float, double>(hipStream_t, int, long, const double*, float*, double*);
+template void root<double, double>(hipStream_t, int, long, const double*, double*, double*);
+
+__global__ void k_carry(int na, const double* __restrict__ PT, const double* __restrict__ seed,
+                        const double* __restrict__ tot, double* __restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < na; ++j) acc = fma(PT[(long)j * na + i], seed[j], acc);
+        out[i] = acc + tot[i];
+    }
+}
+
+void carry(hipStream_t s, int na, const double* P, const double* seed, const double* tot,
+           double* seedOut)
+{
+    hipLaunchKernelGGL(k_carry, dim3((na + 255) / 256), dim3(256), 0, s, na, P, seed, tot, seedOut);
+    check_launch("carry");
+}
+
+// ----------------------------------------------------------------- outputs
+template <typename TX, typename TU>
+__global__ void k_outputs(int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
+                          const float* __restrict__ C, const float* __restrict__ D,
+                          const TX* __restrict__ X, const TU* __restrict__ U, long sO, long oO,
+                          float* __restrict__ Y)
+{
+#pragma clang fp contract(off)
+    using TA = decltype(TX(0) * TU(0));
+    const int nOut = (zC && zD) ? n : p;
+    const long total = N * nOut;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long e = idx / nOut;
+        int j = (int)(idx % nOut);
+        const TX* x = X + e * na + (long)oX * n;
+        if (zC && zD) {
+            Y[idx] = (float)x[j];
+            continue;
+        }
+        TA v = 0;
+        if (!zC)
+            for (int c = 0; c < n; ++c) {
+                TA pr = (TA)C[(long)j * n + c] * (TA)x[c];
+                v = v + pr;
+            }
+        if (!zD && m > 0) {
+            const TU* u = U + (e * sO + oO) * m;
+            for (int c = 0; c < m; ++c) {
+                TA pr = (TA)D[(long)j * m + c] * (TA)u[c];
+                v = v + pr;
+            }
+        }
+        Y[idx] = (float)v;
+    }
+}
+
+template <typename TX, typename TU>
+void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
+             const float* C, const float* D, const TX* X, const TU* U, long sO, long oO, float* Y)
+{
+    long work = N * ((zC && zD) ? n : p);
+    if (work <= 0) return;
+    hipLaunchKernelGGL((k_outputs<TX, TU>), dim3(blocks_for(work, 256, 65536)), dim3(256), 0, s, n,
+                       m, p, na, oX, zC, zD, N, C, D, X, U, sO, oO, Y);
+    check_launch("outputs");
+}
+
+template void outputs<float, float>(hipStream_t, int, int, int, int, int, bool, bool, long,
+                                    const float*, const float*, const float*, const float*, long,
+                                    long, float*);
+template void outputs<float, double>(hipStream_t, int, int, int, int, int, bool, bool, long,
+                                     const float*, const float*, const float*, const double*,
+                                     long, long, float*);
+template void outputs<double, double>(hipStream_t, int, int, int, int, int, bool, bool, long,
+                                      const float*, const float*, const double*, const double*,
+                                      long, long, float*);
+template void outputs<double, float>(hipStream_t, int, int, int, int, int, bool, bool, long,
+                                     const float*, const float*, const double*, const float*,
+                                     long, long, float*);
+
+// ------------------------------------------------------------ ABM start-up
+// Three classic RK4 steps (AdamsMoulton.cpp:96-103).  mirror: float state and
+// float derivative sums (AdamsMoulton.cpp:15-27), stage combinations formed
+// in double and stored to float (odeint runge_kutta4<vector<float>>).
+template <typename TU>
+__global__ void k_abm_startup(bool mirror, int n, int m, double h, const float* __restrict__ A,
+                              const float* __restrict__ B, const float* __restrict__ x0,
+                              const TU* __restrict__ Uh, double* __restrict__ X0)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double sh[];
+    double* x = sh;            // n
+    double* xt = sh + n;       // n
+    double* kb = sh + 2 * n;   // 4n: k1..k4
+    const double dh = h / 2.0, dt6 = h / 6.0, dt3 = h / 3.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        x[r] = (double)x0[r];
+        X0[r] = (double)x0[r];
+    }
+    __syncthreads();
+    auto deriv = [&](const double* xs, int urow, double* out) {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            if (mirror) {
+                float f = 0.0f;
+                for (int c = 0; c < n; ++c) {
+                    float pr = A[(long)r * n + c] * (float)xs[c];
+                    f = f + pr;
+                }
+                for (int c = 0; c < m; ++c) {
+                    float pr = B[(long)r * m + c] * (float)Uh[(long)urow * m + c];
+                    f = f + pr;
+                }
+                out[r] = (double)f;
+            } else {
+                double f = 0.0;
+                for (int c = 0; c < n; ++c) f = fma((double)A[(long)r * n + c], xs[c], f);
+                for (int c = 0; c < m; ++c)
+                    f = fma((double)B[(long)r * m + c], (double)Uh[(long)urow * m + c], f);
+                out[r] = f;
+            }
+        }
+        __syncthreads();
+    };
+    auto stage = [&](const double* kk, double coef) {
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            double v = 1.0 * x[r] + coef * kk[r];
+            xt[r] = mirror ? (double)(float)v : v;
+        }
+        __syncthreads();
+    };
+    for (int s = 0; s < 3; ++s) {
+        double* k1 = kb;
+        double* k2 = kb + n;
+        double* k3 = kb + 2 * n;
+        double* k4 = kb + 3 * n;
+        deriv(x, 2 * s, k1);
+        stage(k1, dh);
+        deriv(xt, 2 * s + 1, k2);
+        stage(k2, dh);
+        deriv(xt, 2 * s + 1, k3);
+        stage(k3, h);
+        deriv(xt, 2 * s + 2, k4);
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            double v;
+            if (mirror)
+                v = (double)(float)((((1.0 * x[r] + dt6 * k1[r]) + dt3 * k2[r]) + dt3 * k3[r]) +
+                                    dt6 * k4[r]);
+            else
+                v = x[r] + (h / 6.0) * (k1[r] + 2.0 * k2[r] + 2.0 * k3[r] + k4[r]);
+            x[r] = v;
+            X0[(long)(s + 1) * n + r] = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename TU>
+void abm_startup(hipStream_t s, bool mirror, int n, int m, double h, const float* A,
+                 const float* B, const float* x0, const TU* Uh, double* X0)
+{
+    size_t lds = (size_t)6 * n * sizeof(double);
+    hipLaunchKernelGGL((k_abm_startup<TU>), dim3(1), dim3(256), lds, s, mirror, n, m, h, A, B, x0,
+                       Uh, X0);
+    check_launch("abm_startup");
+}
+
+template void abm_startup<float>(hipStream_t, bool, int, int, double, const float*, const float*,
+                                 const float*, const float*, double*);
+template void abm_startup<double>(hipStream_t, bool, int, int, double, const float*, const float*,
+                                  const float*, const double*, double*);
+
+template <typename T>
+__global__ void k_to_f64(long count, const T* __restrict__ in, double* __restrict__ out)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+template <typename T>
+void to_f64(hipStream_t s, long count, const T* in, double* out)
+{
+    hipLaunchKernelGGL((k_to_f64<T>), dim3(blocks_for(count, 256, 4096)), dim3(256), 0, s, count, in,
+                       out);
+    check_launch("to_f64");
+}
+
+template void to_f64<float>(hipStream_t, long, const float*, double*);
+
+}  // namespace k
+}  // namespace parode

@@ -1,0 +1,256 @@
+#include "scan_core.h"
+
+#include "kernels.h"
+
+namespace parode {
+
+ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt)
+    : method_(method), n_(n), m_(m), p_(p), na_(method == kABM ? 4 * n : n), zC_(zC), zD_(zD),
+      nPadMax_(next_pow2(nPadMax < 1 ? 1 : nPadMax)), L_(ilog2(nPadMax_)), opt_(opt)
+{
+    if (n <= 0) throw std::invalid_argument("ScanCore: no states");
+    const size_t nn = (size_t)n * n, nm = (size_t)n * (m > 0 ? m : 1);
+    const size_t na2 = (size_t)na_ * na_;
+    const int kq = method == kABM ? 5 : 3;
+    if (opt_.precision == kPlain32) {
+        prod32_.alloc(4 * nn + 4 * nm);            // A, A2, A3, A4 | B, AB, A2B, A3B (float)
+    } else {
+        prod64_.alloc(4 * nn + 4 * nm);
+        A64_.alloc(nn);
+        B64_.alloc(nm);
+    }
+    Bbar64_.alloc((size_t)n * kq * (m > 0 ? m : 1));
+    Bbar32_.alloc((size_t)n * kq * (m > 0 ? m : 1));
+    PT64_.alloc(na2 * (L_ + 1));
+    int n32 = 0;
+    for (int l = 0; l < L_; ++l)
+        if (!level_is_f64(l)) n32 = l + 1;
+    if (n32 > 0) PT32_.alloc(na2 * n32);
+    if (xIsDouble())
+        X64_.alloc((size_t)nPadMax_ * na_);
+    else
+        X32_.alloc((size_t)nPadMax_ * na_);
+    seed_.alloc(na_);
+    tot_.alloc(na_);
+    seedNext_.alloc(na_);
+}
+
+size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt)
+{
+    const long nP = next_pow2(nPadMax < 1 ? 1 : nPadMax);
+    const int L = ilog2(nP);
+    const size_t na = method == kABM ? 4 * n : n;
+    const size_t nn = (size_t)n * n, nm = (size_t)n * (m > 0 ? m : 1);
+    size_t b = (opt.precision == kPlain32 ? 4 : 8) * (4 * nn + 4 * nm);
+    if (opt.precision != kPlain32) b += 8 * (nn + nm);
+    b += 12 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
+    b += 8 * na * na * (L + 1);
+    if (opt.precision != kFp64) b += 4 * na * na * L;
+    b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
+    b += 24 * na;
+    return b;
+}
+
+size_t ScanCore::footprint() const
+{
+    return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
+           Bbar32_.bytes() + PT64_.bytes() + PT32_.bytes() + X32_.bytes() + X64_.bytes() +
+           seed_.bytes() + tot_.bytes() + seedNext_.bytes();
+}
+
+bool ScanCore::level_is_f64(int l) const
+{
+    if (opt_.precision == kFp64) return true;
+    if (opt_.precision == kPlain32) return false;
+    return l >= L_ - opt_.top64;
+}
+
+bool ScanCore::level_is_delta(int l) const
+{
+    return opt_.precision == kMixed && !level_is_f64(l) && l < opt_.deltaLevels;
+}
+
+// ------------------------------------------------------------------ step map
+void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double h)
+{
+    const int n = n_, m = m_;
+    const size_t nn = (size_t)n * n, nm = (size_t)n * (m > 0 ? m : 1);
+    double* MT = PT64_.get();
+    if (opt_.precision == kPlain32) {
+        // float products with sequential-k accumulation, as the reference's host
+        float* A2 = prod32_.get();
+        float* A3 = A2 + nn;
+        float* A4 = A3 + nn;
+        float* AB = A4 + nn;
+        float* A2B = AB + nm;
+        float* A3B = A2B + nm;
+        k::gemm_seq_f32(s, n, n, n, dA, dA, A2);
+        if (method_ == kRK4) {
+            k::gemm_seq_f32(s, n, n, n, A2, dA, A3);
+            k::gemm_seq_f32(s, n, n, n, dA, A3, A4);
+            if (m > 0) {
+                k::gemm_seq_f32(s, n, n, m, dA, dB, AB);
+                k::gemm_seq_f32(s, n, n, m, A2, dB, A2B);
+                k::gemm_seq_f32(s, n, n, m, A3, dB, A3B);
+            }
+            k::rk4_combine(s, true, n, m, h, dA, A2, A3, A4, dB, AB, A2B, A3B, MT, Bbar64_.get(),
+                           Bbar32_.get());
+        } else {
+            if (m > 0) k::gemm_seq_f32(s, n, n, m, dA, dB, AB);
+            k::abm_combine(s, true, n, m, h, dA, A2, dB, AB, MT, Bbar64_.get(), Bbar32_.get());
+        }
+    } else {
+        double* A = A64_.get();
+        double* B = B64_.get();
+        double* A2 = prod64_.get();
+        double* A3 = A2 + nn;
+        double* A4 = A3 + nn;
+        double* AB = A4 + nn;
+        double* A2B = AB + nm;
+        double* A3B = A2B + nm;
+        k::cast_f32_to_f64(s, (long)nn, dA, A);
+        if (m > 0) k::cast_f32_to_f64(s, (long)n * m, dB, B);
+        k::gemm_f64(s, n, n, n, A, A, A2);
+        if (method_ == kRK4) {
+            k::gemm_f64(s, n, n, n, A2, A, A3);
+            k::gemm_f64(s, n, n, n, A, A3, A4);
+            if (m > 0) {
+                k::gemm_f64(s, n, n, m, A, B, AB);
+                k::gemm_f64(s, n, n, m, A2, B, A2B);
+                k::gemm_f64(s, n, n, m, A3, B, A3B);
+            }
+            k::rk4_combine(s, false, n, m, h, A, A2, A3, A4, B, AB, A2B, A3B, MT, Bbar64_.get(),
+                           Bbar32_.get());
+        } else {
+            if (m > 0) k::gemm_f64(s, n, n, m, A, B, AB);
+            k::abm_combine(s, false, n, m, h, A, A2, B, AB, MT, Bbar64_.get(), Bbar32_.get());
+        }
+    }
+}
+
+void ScanCore::build_powers(hipStream_t s)
+{
+    const size_t na2 = (size_t)na_ * na_;
+    // (P^T)^2 = (P^2)^T: squaring the transposed table keeps it transposed
+    for (int l = 0; l < L_; ++l)
+        k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
+                    PT64_.get() + (l + 1) * na2);
+    for (int l = 0; l < L_; ++l)
+        if (!level_is_f64(l))
+            k::make_level_table_f32(s, na_, PT64_.get() + l * na2, level_is_delta(l),
+                                    PT32_.get() + l * na2);
+}
+
+// -------------------------------------------------------------------- leaves
+void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs,
+                      long nReal, long nPad)
+{
+    const int kq = method_ == kABM ? 5 : 3;
+    const int off = method_ == kABM ? 3 * n_ : 0;
+    if (nPad > nPadMax_) throw std::invalid_argument("ScanCore::leaves: batch too long");
+    if (xIsDouble()) {
+        if (uDouble)
+            k::leaves<double, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+                                              (const double*)dU, sL, oL, rs, X64_.get());
+        else
+            k::leaves<double, float, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+                                             (const float*)dU, sL, oL, rs, X64_.get());
+    } else if (uDouble) {
+        k::leaves<float, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+                                         (const double*)dU, sL, oL, rs, X32_.get());
+    } else {
+        k::leaves<float, float, float>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar32_.get(),
+                                       (const float*)dU, sL, oL, rs, X32_.get());
+    }
+}
+
+// --------------------------------------------------------------------- sweeps
+void ScanCore::level(hipStream_t s, bool down, int l, long np)
+{
+    const size_t na2 = (size_t)na_ * na_;
+    const long d = 1L << l;
+    if (xIsDouble()) {
+        k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
+    } else if (level_is_f64(l)) {
+        k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
+    } else {
+        k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
+                               X32_.get());
+    }
+}
+
+// Pairs of level l that intersect the real elements [0, nReal): the others
+// hold only zero leaves (up) or padding states (down) and are skipped.
+static long pairs_at(long nReal, int l) { return (nReal + (2L << l) - 1) / (2L << l); }
+
+void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
+{
+    const int L = ilog2(nPad);
+    for (int l = 0; l < L; ++l) level(s, false, l, pairs_at(nReal, l));
+}
+
+void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
+{
+    const int L = ilog2(nPad);
+    for (int l = L - 1; l >= 0; --l) level(s, true, l, pairs_at(nReal, l));
+}
+
+void ScanCore::root(hipStream_t s, long nPad, const double* dSeed, double* dTot)
+{
+    if (xIsDouble())
+        k::root<double, double>(s, na_, nPad - 1, dSeed, X64_.get(), dTot);
+    else
+        k::root<float, double>(s, na_, nPad - 1, dSeed, X32_.get(), dTot);
+}
+
+void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot,
+                     double* dOut)
+{
+    k::carry(s, na_, powerT(ilog2(nPad)), dSeed, dTot, dOut);
+}
+
+// -------------------------------------------------------------------- outputs
+void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const void* dU,
+                       bool uDouble, long sO, long oO, long N, float* dY)
+{
+    const int oX = method_ == kABM ? 3 : 0;
+    if (xIsDouble()) {
+        if (uDouble)
+            k::outputs<double, double>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X64_.get(),
+                                       (const double*)dU, sO, oO, dY);
+        else
+            k::outputs<double, float>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X64_.get(),
+                                      (const float*)dU, sO, oO, dY);
+    } else if (uDouble) {
+        k::outputs<float, double>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X32_.get(),
+                                  (const double*)dU, sO, oO, dY);
+    } else {
+        k::outputs<float, float>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X32_.get(),
+                                 (const float*)dU, sO, oO, dY);
+    }
+}
+
+void ScanCore::startup_outputs(hipStream_t s, const float* dC, const float* dD, const double* dX0,
+                               const void* dU, bool uDouble, float* dY)
+{
+    // x_0, x_1, x_2 are the first three n-blocks of X0; their inputs are
+    // rows 0, 2, 4 of the h/2-grid start-up samples (AdamsMoulton.cpp:106-116)
+    if (uDouble)
+        k::outputs<double, double>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, dC, dD, dX0,
+                                   (const double*)dU, 2, 0, dY);
+    else
+        k::outputs<double, float>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, dC, dD, dX0,
+                                  (const float*)dU, 2, 0, dY);
+}
+
+void ScanCore::abm_startup(hipStream_t s, const float* dA, const float* dB, const float* dx0,
+                           const void* dUh, bool uDouble, double h, double* dX0)
+{
+    const bool mirror = opt_.precision == kPlain32;
+    if (uDouble)
+        k::abm_startup<double>(s, mirror, n_, m_, h, dA, dB, dx0, (const double*)dUh, dX0);
+    else
+        k::abm_startup<float>(s, mirror, n_, m_, h, dA, dB, dx0, (const float*)dUh, dX0);
+}
+
+}  // namespace parode

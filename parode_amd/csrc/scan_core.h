@@ -1,0 +1,96 @@
+// ScanCore: the device side of one LTI system's parallel-in-time solve.
+//
+// It owns, for one system shape, the step map (M and Bbar), the FP64 power
+// table P_l = M^(2^l) (kept transposed), the per-level tables in the working
+// precision and the trajectory X, and it runs the stages of the reference's
+// scan driver (Sources/GPUFixedStepIterative.cpp:131-466) as asynchronous
+// launches on a caller-supplied stream:
+//   build_map    : DistributeA/M construction (RungeKutta4thOrder.cpp:36-70,
+//                  AdamsMoulton.cpp:238-316) on the device
+//   build_powers : the MM up-sweep products, collapsed to one squaring per level
+//   leaves       : BTimesURK4thOrder / BTimesUAdamsMoulton
+//   sweep_up     : MVV up-sweep (+ left-leaf copy, implicit in place)
+//   root         : root carry with the (I, x0) seed (:294-337)
+//   sweep_down   : MVV down-sweep (+ root copy, in place); the MM down-sweep
+//                  is dead work and is not performed
+//   outputs      : SystemOutput
+#pragma once
+#include "hip_util.h"
+
+namespace parode {
+
+enum Method { kRK4 = 0, kABM = 1 };
+enum Precision { kPlain32 = 0, kMixed = 1, kFp64 = 2 };
+
+struct ScanOptions {
+    int precision = kPlain32;
+    int top64 = 6;        // mixed: the top levels whose products run in FP64
+    int deltaLevels = 0;  // mixed: lowest levels applied as X + (P - I) X
+};
+
+class ScanCore {
+public:
+    ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt);
+
+    int method() const { return method_; }
+    int n() const { return n_; }
+    int m() const { return m_; }
+    int p() const { return p_; }
+    int na() const { return na_; }
+    int nOut() const { return (zC_ && zD_) ? n_ : p_; }
+    int levels() const { return L_; }
+    long nPadMax() const { return nPadMax_; }
+    const ScanOptions& options() const { return opt_; }
+    bool xIsDouble() const { return opt_.precision == kFp64; }
+    size_t footprint() const;
+    static size_t estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt);
+
+    void build_map(hipStream_t s, const float* dA, const float* dB, double h);
+    void build_powers(hipStream_t s);
+    void leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs, long nReal,
+                long nPad);
+    void sweep_up(hipStream_t s, long nReal, long nPad);
+    void root(hipStream_t s, long nPad, const double* dSeed, double* dTot);
+    void sweep_down(hipStream_t s, long nReal, long nPad);
+    // seedOut = M^nPad * seed + tot (the accumulator carried to the next batch)
+    void carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot, double* dOut);
+    void outputs(hipStream_t s, const float* dC, const float* dD, const void* dU, bool uDouble,
+                 long sO, long oO, long N, float* dY);
+    // outputs of the ABM start-up states x_0..x_2 (dX0 holds x_0..x_3)
+    void startup_outputs(hipStream_t s, const float* dC, const float* dD, const double* dX0,
+                         const void* dU, bool uDouble, float* dY);
+    void abm_startup(hipStream_t s, const float* dA, const float* dB, const float* dx0,
+                     const void* dUh, bool uDouble, double h, double* dX0);
+
+    double* seed() const { return seed_.get(); }
+    double* tot() const { return tot_.get(); }
+    double* seedNext() const { return seedNext_.get(); }
+    const double* powerT(int l) const { return PT64_.get() + (size_t)l * na_ * na_; }
+    void* X() const { return xIsDouble() ? (void*)X64_.get() : (void*)X32_.get(); }
+
+private:
+    bool level_is_f64(int l) const;
+    bool level_is_delta(int l) const;
+    void level(hipStream_t s, bool down, int l, long np);
+
+    int method_, n_, m_, p_, na_;
+    bool zC_, zD_;
+    long nPadMax_;
+    int L_;
+    ScanOptions opt_;
+    // step map
+    DevArray<float> prod32_;      // float products of A (mirror)
+    DevArray<double> prod64_;     // FP64 products of A
+    DevArray<double> A64_, B64_;
+    DevArray<double> Bbar64_;
+    DevArray<float> Bbar32_;
+    // power table (transposed), L+1 levels, and FP32 level tables
+    DevArray<double> PT64_;
+    DevArray<float> PT32_;
+    // trajectory
+    DevArray<float> X32_;
+    DevArray<double> X64_;
+    DevArray<double> seed_, tot_, seedNext_;
+};
+
+}  // namespace parode

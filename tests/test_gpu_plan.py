@@ -1,0 +1,73 @@
+"""Device-resident plans (ParODEExt.h) against the C ABI path and the oracle."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from parode_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a, dtype=None):
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+    return torch.from_numpy(a).to("cuda:0")
+
+
+def _run_plan(s, U, precision, p=None):
+    zc = zd = 0
+    plan = api.Plan(s.method, s.n, s.m, s.p, zc, zd, s.steps, s.h, precision,
+                    U.dtype == np.float64)
+    tA, tB, tC, tD, tx = (_dev(v) for v in (s.A, s.B, s.C, s.D, s.x0))
+    tU = _dev(U)
+    tY = torch.zeros((s.steps, s.p), dtype=torch.float32, device="cuda:0")
+    plan.execute(tA.data_ptr(), tB.data_ptr(), tC.data_ptr(), tD.data_ptr(), tx.data_ptr(),
+                 tU.data_ptr(), tY.data_ptr(), 0)
+    torch.cuda.synchronize()
+    y = tY.cpu().numpy()
+    plan.close()
+    return y
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_plan_plain_matches_oracle(solver):
+    s = synth.make_system(solver, 12, 3, 4, -5.0, -0.1, 1e-3, 3000, seed=21)
+    U = synth.samples_f64(s).astype(np.float32)
+    y = _run_plan(s, U, api.PLAIN32)
+    ye = orc.simulate_mirror_exact(solver, s.A, s.B, s.C, s.D, s.x0, 0.0, s.t_end, s.h, Uhalf=U)
+    assert orc.max_rel_err(y, ye) <= 1e-5
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_plan_equals_c_abi_sampled_path(solver):
+    api.initialize_selected_gpus([0])
+    s = synth.make_system(solver, 10, 2, 2, -5.0, -0.1, 1e-3, 1500, seed=22)
+    U = synth.samples_f64(s).astype(np.float32)
+    y = _run_plan(s, U, api.PLAIN32)
+    tv, yc, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h, s.x0)
+    assert e == 0
+    # same kernels, same order: identical results
+    np.testing.assert_array_equal(y, yc)
+    api.close_gpu()
+
+
+@pytest.mark.parametrize("precision", [api.MIXED, api.FP64])
+def test_plan_precision_modes(precision):
+    s = synth.make_system(1, 16, 2, 2, -10.0, -0.05, 5e-4, 1 << 13, seed=23, noise_sigma=0.1)
+    U = synth.samples_f64(s)
+    y = _run_plan(s, U, precision)
+    y64 = orc.simulate_f64(1, s.A.astype(np.float64), s.B.astype(np.float64),
+                           s.C.astype(np.float64), s.D.astype(np.float64),
+                           s.x0.astype(np.float64), U, s.steps, s.h)
+    err = orc.max_rel_err(y, y64)
+    print(f"precision {precision}: max rel err vs FP64 {err:.3e}")
+    assert err <= (1e-7 if precision == api.FP64 else 1e-5)
+
+
+def test_plan_repeated_execution_is_deterministic():
+    s = synth.make_system(0, 32, 4, 4, -5.0, -0.1, 1e-3, 4096, seed=24)
+    U = synth.samples_f64(s).astype(np.float32)
+    y1 = _run_plan(s, U, api.PLAIN32)
+    y2 = _run_plan(s, U, api.PLAIN32)
+    np.testing.assert_array_equal(y1, y2)
