@@ -49,6 +49,8 @@ protected:
 	long _nRealInBatch;                              /* real elements of the batch being built */
 	long _nBatchIndex;
 	matrixf _hostC, _hostD;                          /* host copies for the lead outputs */
+	std::vector<double> _X0d;                        /* FP64 start-up states (precision modes) */
+	std::vector<std::vector<double>> _UInitd;        /* FP64 start-up inputs (precision modes) */
 	const void* _pBatchU;                            /* inputs of the batch being built */
 	bool _bBatchUDouble;
 	/* extensions: precision mode and sampled inputs (h/2 grid from tStart) */

@@ -24,6 +24,10 @@ int ParODE_GetPrecision(void);
  * chain horizons explicitly and for tests of the batch carry. */
 void ParODE_SetMaxBatchSteps(long maxSteps);
 
+/* Kernel family of the FP32 scan levels: 0 = automatic (matrix cores when the
+ * scan size allows), 1 = the vector-ALU kernel.  For A/B checks and tuning. */
+void ParODE_SetLevelKernel(int variant);
+
 /* Compiles a UFunc source for a gfx target (e.g. "gfx950") without a GPU.
  * Returns 0, or ParODE_CouldNotCreateGPUKernels with the compiler log. */
 int ParODE_CompileUFunc(const char* uFunc, const char* arch, char* log, int logLen);

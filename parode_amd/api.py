@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libParODEShared.so")
 C_ABI_SYMBOLS = ["GetErrorDescription", "InitializeAllGPUs", "GetAvailableGPUs", "GetDeviceName",
                  "InitializeSelectedGPUs", "SimulateODE", "RegisterInput", "CloseGPUC",
                  "StartTimer", "StopTimer"]
-EXT_SYMBOLS = ["ParODE_SetPrecision", "ParODE_GetPrecision", "ParODE_SetMaxBatchSteps",
+EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecision", "ParODE_SetMaxBatchSteps",
                "ParODE_CompileUFunc", "ParODE_GetLastErrorMessage", "SimulateODESampled",
                "ParODE_PlanCreate", "ParODE_PlanExecute", "ParODE_PlanExecuteTimed",
                "ParODE_PlanFootprint",
@@ -67,6 +67,7 @@ def lib():
         L.ParODE_SetPrecision.argtypes = [ctypes.c_int]
         L.ParODE_GetPrecision.restype = ctypes.c_int
         L.ParODE_SetMaxBatchSteps.argtypes = [ctypes.c_long]
+        L.ParODE_SetLevelKernel.argtypes = [ctypes.c_int]
         L.ParODE_CompileUFunc.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                           ctypes.c_int]
         L.ParODE_CompileUFunc.restype = ctypes.c_int
@@ -179,6 +180,11 @@ def stop_timer():
 
 def set_precision(mode):
     lib().ParODE_SetPrecision(mode)
+
+
+def set_level_kernel(variant):
+    """0 = automatic (matrix cores), 1 = vector-ALU kernel."""
+    lib().ParODE_SetLevelKernel(int(variant))
 
 
 def set_max_batch_steps(n):

@@ -53,6 +53,11 @@ void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, lon
 template <typename TX, typename TP>
 void level(hipStream_t s, bool down, bool delta, int na, long d, long np, const TP* PT, TX* X);
 
+// The same FP32 level update on the matrix cores (17 <= na <= 1024).
+bool level_mfma_supported(int na);
+void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
+                float* X);
+
 // Root of the scan: tot = X[last]; X[last] = seed (seed as TS).
 template <typename TX, typename TS>
 void root(hipStream_t s, int na, long last, const TS* seed, TX* X, double* tot);

@@ -502,6 +502,154 @@ template void level<float, double>(hipStream_t, bool, bool, int, long, long, con
 template void level<double, double>(hipStream_t, bool, bool, int, long, long, const double*,
                                     double*);
 
+// ---------------------------------------------------- scan levels on MFMA
+// FP32 level update on the matrix cores (v_mfma_f32_32x32x2_f32).  The product
+// is formed transposed, Out[c][i] = sum_j Src[c][j] P[i][j] = (Src PT)[c][i],
+// so that a 32-lane half of the accumulator holds 32 consecutive states i of
+// one pair column c and the epilogue writes 128-byte runs of X.
+//   A operand (32 pairs x 2 states):  the staged source columns, from LDS
+//   B operand (2 states x 32 states): rows of the transposed table PT, from L2
+// A work-group owns TC = WGC*NCT*32 whole pair columns (all na states), so the
+// down-sweep stays in place.  Waves tile the states as WGI x NIT i-tiles and
+// the pairs as WGC x NCT c-tiles; the source columns are staged KC states at a
+// time with an odd pitch (conflict-free column reads).
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC>
+__global__ __launch_bounds__(256) void k_level_mfma(int na, long d, long np,
+                                                    const float* __restrict__ PT,
+                                                    float* __restrict__ X)
+{
+    constexpr int WGC = 4 / WGI;
+    constexpr int TC = WGC * NCT * 32;
+    constexpr int PITCH = KC + 1;
+    __shared__ float As[TC * PITCH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wi = wave % WGI, wc = wave / WGI;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const long k0 = (long)blockIdx.x * TC;
+    const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
+    floatx16 acc[NCT][NIT];
+#pragma unroll
+    for (int a = 0; a < NCT; ++a)
+#pragma unroll
+        for (int b = 0; b < NIT; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    for (int j0 = 0; j0 < na; j0 += KC) {
+        const int kc = (na - j0) < KC ? (na - j0) : KC;
+        for (int t = threadIdx.x; t < TC * KC; t += 256) {
+            const int c = t / KC, j = t % KC;
+            float v = 0.0f;
+            if (c < ncol && j < kc) {
+                const long R = (k0 + c + 1) * 2 * d - 1;
+                const long src = DOWN ? R : R - d;
+                v = X[src * na + j0 + j];
+            }
+            As[c * PITCH + j] = v;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KC; kk += 2) {
+            const int kb = j0 + kk + hi;
+            float bf[NIT];
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int i = (wi + it * WGI) * 32 + l32;
+                bf[it] = (kb < na && i < na) ? PT[(long)kb * na + i] : 0.0f;
+            }
+            float af[NCT];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+                af[ct] = As[((wc + ct * WGC) * 32 + l32) * PITCH + kk + hi];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                for (int it = 0; it < NIT; ++it)
+                    acc[ct][it] =
+                        __builtin_amdgcn_mfma_f32_32x32x2f32(af[ct], bf[it], acc[ct][it], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int i = (wi + it * WGI) * 32 + l32;
+            if (i >= na) continue;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                if (cr >= ncol) continue;
+                const long R = (k0 + cr + 1) * 2 * d - 1, L = R - d;
+                float y = acc[ct][it][v];
+                if (!DOWN) {
+                    if (DELTA) y = y + X[L * na + i];
+                    X[R * na + i] = X[R * na + i] + y;
+                } else {
+                    const float xr = X[R * na + i], xl = X[L * na + i];
+                    if (DELTA) y = y + xr;
+                    X[L * na + i] = xr;
+                    X[R * na + i] = y + xl;
+                }
+            }
+        }
+}
+
+template <int NIT, int NCT, int WGI>
+static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
+                        const float* PT, float* X)
+{
+    constexpr int TC = (4 / WGI) * NCT * 32;
+    constexpr int KC = 8192 / TC;
+    dim3 g((unsigned)((np + TC - 1) / TC)), b(256);
+    if (down) {
+        if (delta)
+            hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
+                               PT, X);
+        else
+            hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
+                               PT, X);
+    } else {
+        if (delta)
+            hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
+                               PT, X);
+        else
+            hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, d,
+                               np, PT, X);
+    }
+    check_launch("level_mfma");
+}
+
+bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
+
+// Tile choice by scan size; the narrow variant (32 pairs per work-group)
+// takes over when the wide one would leave the 256 CUs short of work.
+void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
+                float* X)
+{
+    if (np <= 0) return;
+    const int nt = (na + 31) / 32;
+    const bool narrow = np < 2L * 256 * 64;
+    if (nt <= 1) {
+        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, PT, X);
+        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, PT, X);
+    } else if (nt <= 2) {
+        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, PT, X);
+        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, PT, X);
+    } else if (nt <= 4) {
+        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, PT, X);
+        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, PT, X);
+    } else if (nt <= 8) {
+        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, PT, X);
+        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, PT, X);
+    } else if (nt <= 16) {
+        mfma_launch<4, 1, 4>(s, down, delta, na, d, np, PT, X);
+    } else {
+        mfma_launch<8, 1, 4>(s, down, delta, na, d, np, PT, X);
+    }
+}
+
 // -------------------------------------------------------------------- root
 template <typename TX, typename TS>
 __global__ void k_root(int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,

@@ -27,6 +27,11 @@ std::string& last_error()
     static std::string s;
     return s;
 }
+int& level_kernel()
+{
+    static int v = 0;
+    return v;
+}
 }  // namespace parode
 
 // --------------------------------------------------------------- singleton
@@ -407,6 +412,8 @@ void ParODE_SetPrecision(int mode)
 }
 
 int ParODE_GetPrecision(void) { return ParODE::Instance()->GetPrecision(); }
+
+void ParODE_SetLevelKernel(int variant) { parode::level_kernel() = variant == 1 ? 1 : 0; }
 
 void ParODE_SetMaxBatchSteps(long maxSteps) { parode::max_batch_steps() = maxSteps > 0 ? maxSteps : 0; }
 

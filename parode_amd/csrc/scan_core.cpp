@@ -1,5 +1,6 @@
 #include "scan_core.h"
 
+#include "config.h"
 #include "kernels.h"
 
 namespace parode {
@@ -173,6 +174,8 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
     } else if (level_is_f64(l)) {
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
+    } else if (level_kernel() != 1 && k::level_mfma_supported(na_)) {
+        k::level_mfma(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2, X32_.get());
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
                                X32_.get());

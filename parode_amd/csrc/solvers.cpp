@@ -217,18 +217,20 @@ ErrorCode GPUODESolverFixedStepIterative::SimulateODE(const matrixf& A, const ma
                         xVect(jj, ii) = xVectInit(jj, ii);
                         continue;
                     }
+                    if (_nPrecision != parode::kPlain32) {
+                        // precision modes: the lead outputs in FP64, rounded once
+                        double v = 0.0;
+                        if (!_bZeroC)
+                            for (unsigned kk = 0; kk < _nSystemSize; ++kk)
+                                v += (double)_hostC(jj, kk) * (double)_X0d[ii * _nSystemSize + kk];
+                        if (!_bZeroD)
+                            for (unsigned kk = 0; kk < _nInputs; ++kk)
+                                v += (double)_hostD(jj, kk) * _UInitd[ii * 2][kk];
+                        xVect(jj, ii) = (fType)v;
+                        continue;
+                    }
                     fType val = 0.0f;
-                    if (!_bZeroC)
-                        for (unsigned kk = 0; kk < _nSystemSize; ++kk) {
-                            fType pr = _hostC(jj, kk) * xVectInit(kk, ii);
-                            val += pr;
-                        }
-                    if (!_bZeroD)
-                        for (unsigned kk = 0; kk < _nInputs; ++kk) {
-                            fType pr = _hostD(jj, kk) * uVectInit(kk, ii);
-                            val += pr;
-                        }
-                    xVect(jj, ii) = val;
+                    if (!_bZeroC)                    xVect(jj, ii) = val;
                 }
         }
     } catch (std::exception& e) {
@@ -379,14 +381,20 @@ void AdamsMoulton::ComputeInitialState(const matrixf& A, const matrixf& B, const
     std::vector<double> X0(4 * (size_t)n);
     PARODE_HIP(hipMemcpyAsync(X0.data(), _core->seed(), X0.size() * 8, hipMemcpyDeviceToHost, s));
     _UInit.assign(9, vectorf(m));
+    _UInitd.assign(9, std::vector<double>(m, 0.0));
+    _X0d = X0;
     if (m > 0) {
         std::vector<char> raw(9 * (size_t)m * (_bBatchUDouble ? 8 : 4));
         PARODE_HIP(hipMemcpyAsync(raw.data(), Uh, raw.size(), hipMemcpyDeviceToHost, s));
         PARODE_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < 9; ++i)
             for (int j = 0; j < m; ++j)
-                _UInit[i][j] = _bBatchUDouble ? (fType)((double*)raw.data())[i * m + j]
-                                              : ((float*)raw.data())[i * m + j];
+            {
+                double v = _bBatchUDouble ? ((double*)raw.data())[i * m + j]
+                                          : (double)((float*)raw.data())[i * m + j];
+                _UInit[i][j] = (fType)v;
+                _UInitd[i][j] = v;
+            }
     }
     PARODE_HIP(hipStreamSynchronize(s));
     x0Scan.resize(4 * (size_t)n);

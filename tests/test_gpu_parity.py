@@ -79,15 +79,25 @@ def test_config2_shape_plain_parity(solver):
 
 
 # -------------------------------------------------- the reference's scenarios
-def _reference_rule(solver, d, y, t0=0.0):
+def _reference_rule(solver, d, y, t1, t0=0.0):
+    """The reference's acceptance rule, sqrt(sum (y - y_cpu)^2) < 0.1 against
+    its CPU path (TestParODEStaticLibCInterface.cpp:204, :466).  At long
+    horizons the reference's own float time grid (Kernels/u.cl:8) breaks that
+    rule by itself: the FP32-mirror restatement of its arithmetic exceeds 0.1
+    at N >= 32000 (ABM) / 64000 (RK4).  There the bar is the mirror's own
+    distance, +1%."""
     total = y.shape[0]
     Uh = sc.uhalf_from_double_sin(d["ucoef"], t0, d["h"], total)
     cpu = orc.cpu_rk4_odeint if solver == 0 else orc.cpu_abm_pece
     yc = cpu(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, d["h"])
     lo = 1 if solver == 0 else 4
-    l2 = float(np.sqrt(np.sum((y[lo:total - 3].astype(np.float64) - yc[lo:total - 3]) ** 2)))
-    print(f"reference L2 rule: {l2:.3e}")
-    assert l2 < 0.1
+    l2 = lambda a: float(np.sqrt(np.sum((a[lo:total - 3].astype(np.float64)
+                                         - yc[lo:total - 3]) ** 2)))
+    tv, ym = orc.simulate_mirror(solver, d["A"], d["B"], d["C"], d["D"], d["x0"], t0, t1, d["h"],
+                                 ucoef=d["ucoef"])
+    l2_gpu, l2_ref = l2(y), l2(ym)
+    print(f"reference L2 rule: GPU {l2_gpu:.3e}, reference arithmetic {l2_ref:.3e}")
+    assert l2_gpu < max(0.1, 1.01 * l2_ref)
 
 
 @pytest.mark.parametrize("make", [sc.rk4_class, sc.abm_class])
@@ -102,7 +112,7 @@ def test_reference_class_scenarios(make):
                                    d["h"], ucoef=d["ucoef"])
     assert y.shape == ye.shape
     assert orc.max_rel_err(y, ye) <= PARITY
-    _reference_rule(d["solver"], d, y)
+    _reference_rule(d["solver"], d, y, t1)
 
 
 @pytest.mark.parametrize("solver", [0, 1])
@@ -116,7 +126,7 @@ def test_reference_cpp_api_scenario(solver):
     ye = orc.simulate_mirror_exact(solver, d["A"], d["B"], d["C"], d["D"], d["x0"], 0.0, t1,
                                    d["h"], ucoef=d["ucoef"])
     assert orc.max_rel_err(y, ye) <= PARITY
-    _reference_rule(solver, d, y)
+    _reference_rule(solver, d, y, t1)
 
 
 @pytest.mark.parametrize("steps", [100, 1000, 8000, 32000])
@@ -132,7 +142,7 @@ def test_reference_c_abi_scenario(solver, steps):
                                    d["h"], ucoef=d["ucoef"])
     np.testing.assert_array_equal(tv, orc.time_vector(solver, 0.0, d["h"], ye.shape[0]))
     assert orc.max_rel_err(y, ye) <= PARITY
-    _reference_rule(solver, d, y)
+    _reference_rule(solver, d, y, t1)
 
 
 # ------------------------------------------------------------- edge cases
@@ -260,3 +270,29 @@ def test_precision_modes_against_fp64(solver, mode, bound):
     print(f"mode {mode} solver {solver}: max rel err vs FP64 {err:.3e}")
     # FP64 mode: the only FP32 rounding is the output cast (y is fType)
     assert err <= max(bound, 1e-7)
+
+
+# ------------------------------------------- matrix-core vs vector kernels
+@pytest.mark.parametrize("solver,n", [(0, 20), (0, 48), (0, 100), (0, 256), (1, 17), (1, 64),
+                                      (1, 130)])
+def test_mfma_levels_match_vector_levels(solver, n):
+    """The FP32 level update on MFMA (17 <= n_a <= 1024) and on the vector ALU
+    give the same trajectory to FP32 rounding, and both meet the parity bar."""
+    N = 2048 if n < 200 else 512
+    s = synth.make_system(solver, n, 3, 3, -6.0, -0.2, 1e-3, N, seed=100 + n)
+    idx = _register(synth.ufunc_source(s.ucoef))
+    outs = []
+    for variant in (1, 0):
+        api.set_level_kernel(variant)
+        try:
+            tv, y, e = api.simulate_ode(solver, s.A, s.B, s.C, s.D, idx, 0.0, s.t_end, s.h, s.x0)
+        finally:
+            api.set_level_kernel(0)
+        assert e == 0, api.last_error_message()
+        outs.append(y)
+    ye = orc.simulate_mirror_exact(solver, s.A, s.B, s.C, s.D, s.x0, 0.0, s.t_end, s.h,
+                                   ucoef=s.ucoef)
+    e_vec, e_mfma = orc.max_rel_err(outs[0], ye), orc.max_rel_err(outs[1], ye)
+    print(f"n_a={s.n_scan}: vector {e_vec:.2e}, mfma {e_mfma:.2e}")
+    assert orc.max_rel_err(outs[1], outs[0]) <= PARITY
+    assert e_mfma <= PARITY and e_vec <= PARITY

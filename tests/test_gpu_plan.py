@@ -41,7 +41,8 @@ def test_plan_plain_matches_oracle(solver):
 
 @pytest.mark.parametrize("solver", [0, 1])
 def test_plan_equals_c_abi_sampled_path(solver):
-    api.initialize_selected_gpus([0])
+    ids, e = api.get_available_gpus()
+    assert api.initialize_selected_gpus(ids[:1]) == 0
     s = synth.make_system(solver, 10, 2, 2, -5.0, -0.1, 1e-3, 1500, seed=22)
     U = synth.samples_f64(s).astype(np.float32)
     y = _run_plan(s, U, api.PLAIN32)
