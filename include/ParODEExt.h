@@ -64,7 +64,8 @@ int ParODE_PlanExecute(void* plan, const float* dA, const float* dB, const float
 					   float* dY, void* stream);
 /* As ParODE_PlanExecute, then waits for the solve and writes the GPU time of
  * its phases to phaseMs[6]: step map + FP64 powers, leaves (and ABM start-up),
- * up-sweep, root, down-sweep, outputs (hipEvents on `stream`).  For
+ * up-sweep launches, the top of the tree with the root (one work-group),
+ * down-sweep launches, outputs (hipEvents on `stream`).  For
  * benchmarks and profilers. */
 int ParODE_PlanExecuteTimed(void* plan, const float* dA, const float* dB, const float* dC,
 							const float* dD, const float* dx0, const void* dSamples,

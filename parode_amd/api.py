@@ -274,7 +274,7 @@ class Plan:
         if r != 0:
             raise ParODEError(r, "ParODE_PlanExecute")
 
-    PHASES = ("map_powers", "leaves", "up_sweep", "root", "down_sweep", "outputs")
+    PHASES = ("map_powers", "leaves", "up_sweep", "top", "down_sweep", "outputs")
 
     def execute_timed(self, dA, dB, dC, dD, dx0, dU, dY, stream=0):
         """Execute, wait, and return {phase: GPU ms} measured with hipEvents."""

@@ -57,7 +57,7 @@ void run_plan(Plan* pl, const float* dA, const float* dB, const float* dC, const
     tick(kPhaseUp);
     c.sweep_up(s, pl->nScan, pl->nPad);
     tick(kPhaseRoot);
-    c.root(s, pl->nPad, c.seed(), c.tot());
+    c.sweep_top(s, pl->nScan, pl->nPad, c.seed(), c.tot());
     tick(kPhaseDown);
     c.sweep_down(s, pl->nScan, pl->nPad);
     tick(kPhaseOutputs);

@@ -19,7 +19,10 @@ namespace k {
 // no fused multiply-add (the rounding of uBLAS prod() on matrix<float>).
 void gemm_seq_f32(hipStream_t s, int r, int kk, int c, const float* X, const float* Y, float* Z);
 // FP64 tiled GEMM, Z = X * Y (used for A-products and the power squarings).
-void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z);
+void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
+              bool mfma = true);
+template <typename T>
+void transpose(hipStream_t s, int rows, int cols, const T* in, T* out);
 void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out);
 void fill_f64(hipStream_t s, long count, double v, double* out);
 
@@ -57,6 +60,12 @@ void level(hipStream_t s, bool down, bool delta, int na, long d, long np, const 
 bool level_mfma_supported(int na);
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
                 float* X);
+
+// Levels [l0, L) of both sweeps and the root in one work-group (levels >= l64
+// from the FP64 table, the others from the FP32 one).
+template <typename TX>
+void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float* PT32,
+         const double* PT64, TX* X, const double* seed, double* tot);
 
 // Root of the scan: tot = X[last]; X[last] = seed (seed as TS).
 template <typename TX, typename TS>

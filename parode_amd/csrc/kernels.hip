@@ -31,6 +31,31 @@ static inline unsigned blocks_for(long work, int block, long cap = 1L << 20)
     return (unsigned)std::min(b, cap);
 }
 
+// ---------------------------------------------------------------- helpers
+template <typename T>
+__global__ void k_transpose(int rows, int cols, const T* __restrict__ in, T* __restrict__ out)
+{
+    const long total = (long)rows * cols;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long c = idx / rows, r = idx % rows;   // out[c][r] = in[r][c]
+        out[idx] = in[r * cols + c];
+    }
+}
+
+template <typename T>
+void transpose(hipStream_t s, int rows, int cols, const T* in, T* out)
+{
+    const long work = (long)rows * cols;
+    if (work <= 0) return;
+    hipLaunchKernelGGL((k_transpose<T>), dim3(blocks_for(work, 256, 4096)), dim3(256), 0, s, rows,
+                       cols, in, out);
+    check_launch("transpose");
+}
+
+template void transpose<float>(hipStream_t, int, int, const float*, float*);
+template void transpose<double>(hipStream_t, int, int, const double*, double*);
+
 // ------------------------------------------------------------------ GEMMs
 __global__ void k_gemm_seq_f32(int r, int kk, int c, const float* __restrict__ X,
                                const float* __restrict__ Y, float* __restrict__ Z)
@@ -105,11 +130,61 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int r, int kk, int c,
         }
 }
 
-void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z)
+__global__ void k_gemm_f64_mfma(int r, int kk, int c, const double* __restrict__ X,
+                                const double* __restrict__ Y, double* __restrict__ Z);
+
+void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
+              bool mfma)
 {
-    dim3 grid((c + 63) / 64, (r + 63) / 64);
-    hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    if (mfma) {
+        dim3 grid((c + 31) / 32, (r + 31) / 32);
+        hipLaunchKernelGGL(k_gemm_f64_mfma, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    } else {
+        dim3 grid((c + 63) / 64, (r + 63) / 64);
+        hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    }
     check_launch("gemm_f64");
+}
+
+// FP64 GEMM on the matrix cores (v_mfma_f64_16x16x4_f64): a work-group of four
+// waves owns a 32x32 tile of Z, one 16x16 quarter per wave; 32-deep slices of
+// X and Y are staged through LDS.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ Y,
+                                                       double* __restrict__ Z)
+{
+    constexpr int T = 32, KB = 32;
+    __shared__ double As[T][KB + 1];
+    __shared__ double Bs[KB][T + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
+    const int l16 = lane & 15, q4 = lane >> 4;
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kk; k0 += KB) {
+        for (int t = threadIdx.x; t < T * KB; t += 256) {
+            const int a = t / KB, b = t % KB;
+            As[a][b] = (i0 + a < r && k0 + b < kk) ? X[(long)(i0 + a) * kk + k0 + b] : 0.0;
+            const int bk = t / T, bj = t % T;
+            Bs[bk][bj] = (k0 + bk < kk && j0 + bj < c) ? Y[(long)(k0 + bk) * c + j0 + bj] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < KB; k4 += 4) {
+            const double a = As[wr * 16 + l16][k4 + q4];
+            const double b = Bs[k4 + q4][wc * 16 + l16];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wr * 16 + 4 * q4 + v, j = j0 + wc * 16 + l16;
+        if (i < r && j < c) Z[(long)i * c + j] = acc[v];
+    }
 }
 
 __global__ void k_cast_f32_f64(long count, const float* __restrict__ in, double* __restrict__ out)
@@ -341,25 +416,28 @@ struct Acc {
 };
 
 template <typename T, typename TU, typename TB>
-__global__ void k_leaves(int n, int m, int kq, int na, int off, long nReal, long nPad,
-                         const TB* __restrict__ Bbar, const TU* __restrict__ U, long sL, long oL,
-                         long rs, T* __restrict__ X)
+__global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
+                                                long nPad, const TB* __restrict__ BbT,
+                                                const TU* __restrict__ U, long sL, long oL,
+                                                long rs, T* __restrict__ X)
 {
 #pragma clang fp contract(off)
+    // BbT is Bbar transposed, [kq*m][n]: consecutive threads take consecutive
+    // rows r of one element e, so their Bbar reads are contiguous and their
+    // input reads are one broadcast.
     using TA = decltype(TB(0) * TU(0));
     const long total = nPad * na;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (long)gridDim.x * blockDim.x) {
-        long e = idx / na;
-        int rho = (int)(idx % na);
+        const long e = idx / na;
+        const int r = (int)(idx % na) - off;
         TA v = 0;
-        int r = rho - off;
         if (e < nReal && r >= 0 && r < n && m > 0) {
-            const TB* brow = Bbar + (long)r * kq * m;
             for (int q = 0; q < kq; ++q) {
                 const TU* u = U + (e * sL + oL + (long)q * rs) * m;
+                const TB* b = BbT + (long)q * m * n + r;
                 for (int j = 0; j < m; ++j) {
-                    TA pr = (TA)brow[q * m + j] * (TA)u[j];
+                    TA pr = (TA)b[(long)j * n] * (TA)u[j];
                     v = v + pr;
                 }
             }
@@ -650,6 +728,120 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
     }
 }
 
+// ------------------------------------------------------- top of the tree
+// Levels [l0, L) of the up-sweep, the root and the same levels of the
+// down-sweep in ONE work-group of 1024 threads: the top of the tree holds few
+// pairs per level, so one work-group does the work of 2(L - l0) + 1 launches
+// with only barriers between levels.  Levels >= l64 use the FP64 table.
+// Thread t owns state i = t mod rowThreads and the column group t / rowThreads.
+constexpr int kTopThreads = 1024;
+constexpr int kTopMaxCols = 8;
+
+// One level's update of cc staged columns for the rows of thread (i0, g).
+template <typename TX, typename TA>
+__device__ __forceinline__ void top_update(bool down, int na, long d, long c0, int cc, int i0,
+                                           int rowThreads, int g, int groups,
+                                           const TA* __restrict__ P, const double* Bsh, TX* X)
+{
+    for (int i = i0; i < na; i += rowThreads) {
+        TA acc[kTopMaxCols];
+#pragma unroll
+        for (int q = 0; q < kTopMaxCols; ++q) acc[q] = 0;
+        for (int j = 0; j < na; ++j) {
+            const TA pj = P[(long)j * na + i];
+#pragma unroll
+            for (int q = 0; q < kTopMaxCols; ++q) {
+                const int c = g + q * groups;
+                if (c < cc) acc[q] = fma(pj, (TA)Bsh[c * na + j], acc[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kTopMaxCols; ++q) {
+            const int c = g + q * groups;
+            if (c >= cc) continue;
+            const long R = (c0 + c + 1) * 2 * d - 1, Lp = R - d;
+            if (!down) {
+                X[R * na + i] = (TX)((TA)X[R * na + i] + acc[q]);
+            } else {
+                const TA xl = (TA)X[Lp * na + i];
+                X[Lp * na + i] = (TX)Bsh[c * na + i];
+                X[R * na + i] = (TX)(acc[q] + xl);
+            }
+        }
+    }
+}
+
+template <typename TX>
+__global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int l64, long nReal,
+                                                     const float* __restrict__ PT32,
+                                                     const double* __restrict__ PT64,
+                                                     TX* __restrict__ X,
+                                                     const double* __restrict__ seed,
+                                                     double* __restrict__ tot, int CC)
+{
+    extern __shared__ double Bsh[];   // CC x na staged source columns
+    const int t = threadIdx.x;
+    const int rowThreads = na < kTopThreads ? na : kTopThreads;
+    const int groups = kTopThreads / rowThreads;
+    const int i0 = t % rowThreads, g = t / rowThreads;
+    const long na2 = (long)na * na;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool down = pass == 1;
+        for (int li = 0; li < L - l0; ++li) {
+            const int l = down ? L - 1 - li : l0 + li;
+            const long d = 1L << l;
+            const long np = (nReal + (2L << l) - 1) / (2L << l);
+            for (long c0 = 0; c0 < np; c0 += CC) {
+                const int cc = (np - c0) < CC ? (int)(np - c0) : CC;
+                for (int e = t; e < cc * na; e += kTopThreads) {
+                    const int c = e / na, j = e % na;
+                    const long R = (c0 + c + 1) * 2 * d - 1;
+                    Bsh[c * na + j] = (double)X[(down ? R : R - d) * na + j];
+                }
+                __syncthreads();
+                if (g < groups) {
+                    if (l >= l64)
+                        top_update<TX, double>(down, na, d, c0, cc, i0, rowThreads, g, groups,
+                                               PT64 + (long)l * na2, Bsh, X);
+                    else
+                        top_update<TX, float>(down, na, d, c0, cc, i0, rowThreads, g, groups,
+                                              PT32 + (long)l * na2, Bsh, X);
+                }
+                __syncthreads();
+            }
+        }
+        if (!down) {
+            const long last = (1L << L) - 1;
+            for (int i = t; i < na; i += kTopThreads) {
+                if (tot) tot[i] = (double)X[last * na + i];
+                X[last * na + i] = (TX)seed[i];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename TX>
+void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float* PT32,
+         const double* PT64, TX* X, const double* seed, double* tot)
+{
+    // columns per pass: what the threads can hold (groups x kTopMaxCols) and
+    // what 48 KiB of LDS can stage
+    const int rowThreads = na < kTopThreads ? na : kTopThreads;
+    const int groups = kTopThreads / rowThreads;
+    int CC = std::min(groups * kTopMaxCols, 6144 / na);
+    if (CC < 1) CC = 1;
+    size_t lds = (size_t)CC * na * sizeof(double);
+    hipLaunchKernelGGL((k_top<TX>), dim3(1), dim3(kTopThreads), lds, s, na, l0, L, l64, nReal,
+                       PT32, PT64, X, seed, tot, CC);
+    check_launch("top");
+}
+
+template void top<float>(hipStream_t, int, int, int, int, long, const float*, const double*,
+                         float*, const double*, double*);
+template void top<double>(hipStream_t, int, int, int, int, long, const float*, const double*,
+                          double*, const double*, double*);
+
 // -------------------------------------------------------------------- root
 template <typename TX, typename TS>
 __global__ void k_root(int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,
@@ -690,19 +882,23 @@ void carry(hipStream_t s, int na, const double* P, const double* seed, const dou
 
 // ----------------------------------------------------------------- outputs
 template <typename TX, typename TU>
-__global__ void k_outputs(int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
-                          const float* __restrict__ C, const float* __restrict__ D,
-                          const TX* __restrict__ X, const TU* __restrict__ U, long sO, long oO,
-                          float* __restrict__ Y)
+__global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, int oX, bool zC,
+                                                 bool zD, long N, const float* __restrict__ CT,
+                                                 const float* __restrict__ DT,
+                                                 const TX* __restrict__ X,
+                                                 const TU* __restrict__ U, long sO, long oO,
+                                                 float* __restrict__ Y)
 {
 #pragma clang fp contract(off)
+    // CT = C^T ([n][p]) and DT = D^T ([m][p]): the threads of one element read
+    // one contiguous row of each per term.
     using TA = decltype(TX(0) * TU(0));
     const int nOut = (zC && zD) ? n : p;
     const long total = N * nOut;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (long)gridDim.x * blockDim.x) {
-        long e = idx / nOut;
-        int j = (int)(idx % nOut);
+        const long e = idx / nOut;
+        const int j = (int)(idx % nOut);
         const TX* x = X + e * na + (long)oX * n;
         if (zC && zD) {
             Y[idx] = (float)x[j];
@@ -711,13 +907,13 @@ __global__ void k_outputs(int n, int m, int p, int na, int oX, bool zC, bool zD,
         TA v = 0;
         if (!zC)
             for (int c = 0; c < n; ++c) {
-                TA pr = (TA)C[(long)j * n + c] * (TA)x[c];
+                TA pr = (TA)CT[(long)c * p + j] * (TA)x[c];
                 v = v + pr;
             }
         if (!zD && m > 0) {
             const TU* u = U + (e * sO + oO) * m;
             for (int c = 0; c < m; ++c) {
-                TA pr = (TA)D[(long)j * m + c] * (TA)u[c];
+                TA pr = (TA)DT[(long)c * p + j] * (TA)u[c];
                 v = v + pr;
             }
         }

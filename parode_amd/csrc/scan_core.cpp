@@ -22,6 +22,10 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     }
     Bbar64_.alloc((size_t)n * kq * (m > 0 ? m : 1));
     Bbar32_.alloc((size_t)n * kq * (m > 0 ? m : 1));
+    BbT64_.alloc((size_t)n * kq * (m > 0 ? m : 1));
+    BbT32_.alloc((size_t)n * kq * (m > 0 ? m : 1));
+    CT_.alloc((size_t)n * (p > 0 ? p : 1));
+    DT_.alloc((size_t)(m > 0 ? m : 1) * (p > 0 ? p : 1));
     PT64_.alloc(na2 * (L_ + 1));
     int n32 = 0;
     for (int l = 0; l < L_; ++l)
@@ -44,7 +48,7 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     const size_t nn = (size_t)n * n, nm = (size_t)n * (m > 0 ? m : 1);
     size_t b = (opt.precision == kPlain32 ? 4 : 8) * (4 * nn + 4 * nm);
     if (opt.precision != kPlain32) b += 8 * (nn + nm);
-    b += 12 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
+    b += 24 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
     b += 8 * na * na * (L + 1);
     if (opt.precision != kFp64) b += 4 * na * na * L;
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
@@ -55,8 +59,9 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
 size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
-           Bbar32_.bytes() + PT64_.bytes() + PT32_.bytes() + X32_.bytes() + X64_.bytes() +
-           seed_.bytes() + tot_.bytes() + seedNext_.bytes();
+           Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + CT_.bytes() + DT_.bytes() +
+           PT64_.bytes() + PT32_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
+           tot_.bytes() + seedNext_.bytes();
 }
 
 bool ScanCore::level_is_f64(int l) const
@@ -101,6 +106,7 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
             k::abm_combine(s, true, n, m, h, dA, A2, dB, AB, MT, Bbar64_.get(), Bbar32_.get());
         }
     } else {
+        const bool mm = level_kernel() != 1;
         double* A = A64_.get();
         double* B = B64_.get();
         double* A2 = prod64_.get();
@@ -111,21 +117,26 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
         double* A3B = A2B + nm;
         k::cast_f32_to_f64(s, (long)nn, dA, A);
         if (m > 0) k::cast_f32_to_f64(s, (long)n * m, dB, B);
-        k::gemm_f64(s, n, n, n, A, A, A2);
+        k::gemm_f64(s, n, n, n, A, A, A2, mm);
         if (method_ == kRK4) {
-            k::gemm_f64(s, n, n, n, A2, A, A3);
-            k::gemm_f64(s, n, n, n, A, A3, A4);
+            k::gemm_f64(s, n, n, n, A2, A, A3, mm);
+            k::gemm_f64(s, n, n, n, A, A3, A4, mm);
             if (m > 0) {
-                k::gemm_f64(s, n, n, m, A, B, AB);
-                k::gemm_f64(s, n, n, m, A2, B, A2B);
-                k::gemm_f64(s, n, n, m, A3, B, A3B);
+                k::gemm_f64(s, n, n, m, A, B, AB, mm);
+                k::gemm_f64(s, n, n, m, A2, B, A2B, mm);
+                k::gemm_f64(s, n, n, m, A3, B, A3B, mm);
             }
             k::rk4_combine(s, false, n, m, h, A, A2, A3, A4, B, AB, A2B, A3B, MT, Bbar64_.get(),
                            Bbar32_.get());
         } else {
-            if (m > 0) k::gemm_f64(s, n, n, m, A, B, AB);
+            if (m > 0) k::gemm_f64(s, n, n, m, A, B, AB, mm);
             k::abm_combine(s, false, n, m, h, A, A2, B, AB, MT, Bbar64_.get(), Bbar32_.get());
         }
+    }
+    const int kq = method_ == kABM ? 5 : 3;
+    if (m > 0) {
+        k::transpose<double>(s, n, kq * m, Bbar64_.get(), BbT64_.get());
+        k::transpose<float>(s, n, kq * m, Bbar32_.get(), BbT32_.get());
     }
 }
 
@@ -133,9 +144,10 @@ void ScanCore::build_powers(hipStream_t s)
 {
     const size_t na2 = (size_t)na_ * na_;
     // (P^T)^2 = (P^2)^T: squaring the transposed table keeps it transposed
+    const bool mm = level_kernel() != 1;
     for (int l = 0; l < L_; ++l)
         k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
-                    PT64_.get() + (l + 1) * na2);
+                    PT64_.get() + (l + 1) * na2, mm);
     for (int l = 0; l < L_; ++l)
         if (!level_is_f64(l))
             k::make_level_table_f32(s, na_, PT64_.get() + l * na2, level_is_delta(l),
@@ -151,16 +163,16 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
     if (nPad > nPadMax_) throw std::invalid_argument("ScanCore::leaves: batch too long");
     if (xIsDouble()) {
         if (uDouble)
-            k::leaves<double, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+            k::leaves<double, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
                                               (const double*)dU, sL, oL, rs, X64_.get());
         else
-            k::leaves<double, float, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+            k::leaves<double, float, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
                                              (const float*)dU, sL, oL, rs, X64_.get());
     } else if (uDouble) {
-        k::leaves<float, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar64_.get(),
+        k::leaves<float, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
                                          (const double*)dU, sL, oL, rs, X32_.get());
     } else {
-        k::leaves<float, float, float>(s, n_, m_, kq, na_, off, nReal, nPad, Bbar32_.get(),
+        k::leaves<float, float, float>(s, n_, m_, kq, na_, off, nReal, nPad, BbT32_.get(),
                                        (const float*)dU, sL, oL, rs, X32_.get());
     }
 }
@@ -186,24 +198,45 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
 // hold only zero leaves (up) or padding states (down) and are skipped.
 static long pairs_at(long nReal, int l) { return (nReal + (2L << l) - 1) / (2L << l); }
 
-void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
+int ScanCore::top_level(long nReal, long nPad) const
 {
     const int L = ilog2(nPad);
-    for (int l = 0; l < L; ++l) level(s, false, l, pairs_at(nReal, l));
+    if (level_kernel() == 1) return L;               // vector kernels only: no fused top
+    if ((long)nPad * na_ <= 65536) return 0;         // small: the whole tree in one work-group
+    int l = 0;
+    while (l < L && pairs_at(nReal, l) > 32) ++l;
+    return l;
+}
+
+void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
+{
+    const int lt = top_level(nReal, nPad);
+    for (int l = 0; l < lt; ++l) level(s, false, l, pairs_at(nReal, l));
+}
+
+void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSeed, double* dTot)
+{
+    const int L = ilog2(nPad), lt = top_level(nReal, nPad);
+    if (lt >= L) {   // no fused top: the root alone
+        if (xIsDouble())
+            k::root<double, double>(s, na_, nPad - 1, dSeed, X64_.get(), dTot);
+        else
+            k::root<float, double>(s, na_, nPad - 1, dSeed, X32_.get(), dTot);
+        return;
+    }
+    int l64 = L;
+    while (l64 > 0 && level_is_f64(l64 - 1)) --l64;
+    if (xIsDouble())
+        k::top<double>(s, na_, lt, L, 0, nReal, nullptr, PT64_.get(), X64_.get(), dSeed, dTot);
+    else
+        k::top<float>(s, na_, lt, L, l64, nReal, PT32_.get(), PT64_.get(), X32_.get(), dSeed,
+                      dTot);
 }
 
 void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
 {
-    const int L = ilog2(nPad);
-    for (int l = L - 1; l >= 0; --l) level(s, true, l, pairs_at(nReal, l));
-}
-
-void ScanCore::root(hipStream_t s, long nPad, const double* dSeed, double* dTot)
-{
-    if (xIsDouble())
-        k::root<double, double>(s, na_, nPad - 1, dSeed, X64_.get(), dTot);
-    else
-        k::root<float, double>(s, na_, nPad - 1, dSeed, X32_.get(), dTot);
+    const int lt = top_level(nReal, nPad);
+    for (int l = lt - 1; l >= 0; --l) level(s, true, l, pairs_at(nReal, l));
 }
 
 void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot,
@@ -217,6 +250,11 @@ void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const vo
                        bool uDouble, long sO, long oO, long N, float* dY)
 {
     const int oX = method_ == kABM ? 3 : 0;
+    const bool zC = zC_ || !dC, zD = zD_ || !dD;
+    if (!zC) k::transpose<float>(s, p_, n_, dC, CT_.get());
+    if (!zD && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
+    dC = CT_.get();
+    dD = DT_.get();
     if (xIsDouble()) {
         if (uDouble)
             k::outputs<double, double>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X64_.get(),
@@ -238,11 +276,13 @@ void ScanCore::startup_outputs(hipStream_t s, const float* dC, const float* dD, 
 {
     // x_0, x_1, x_2 are the first three n-blocks of X0; their inputs are
     // rows 0, 2, 4 of the h/2-grid start-up samples (AdamsMoulton.cpp:106-116)
+    if (!zC_) k::transpose<float>(s, p_, n_, dC, CT_.get());
+    if (!zD_ && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
     if (uDouble)
-        k::outputs<double, double>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, dC, dD, dX0,
+        k::outputs<double, double>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, CT_.get(), DT_.get(), dX0,
                                    (const double*)dU, 2, 0, dY);
     else
-        k::outputs<double, float>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, dC, dD, dX0,
+        k::outputs<double, float>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, CT_.get(), DT_.get(), dX0,
                                   (const float*)dU, 2, 0, dY);
 }
 

@@ -10,7 +10,8 @@
 //   build_powers : the MM up-sweep products, collapsed to one squaring per level
 //   leaves       : BTimesURK4thOrder / BTimesUAdamsMoulton
 //   sweep_up     : MVV up-sweep (+ left-leaf copy, implicit in place)
-//   root         : root carry with the (I, x0) seed (:294-337)
+//   sweep_top    : the top of the tree and the root carry with the (I, x0)
+//                  seed (:294-337) in one work-group
 //   sweep_down   : MVV down-sweep (+ root copy, in place); the MM down-sweep
 //                  is dead work and is not performed
 //   outputs      : SystemOutput
@@ -49,8 +50,12 @@ public:
     void build_powers(hipStream_t s);
     void leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs, long nReal,
                 long nPad);
+    // The scan of one batch runs as: up-sweep launches for the levels below
+    // top_level(), one work-group for the top of the tree (its up levels, the
+    // root with the seed, its down levels), down-sweep launches below.
+    int top_level(long nReal, long nPad) const;
     void sweep_up(hipStream_t s, long nReal, long nPad);
-    void root(hipStream_t s, long nPad, const double* dSeed, double* dTot);
+    void sweep_top(hipStream_t s, long nReal, long nPad, const double* dSeed, double* dTot);
     void sweep_down(hipStream_t s, long nReal, long nPad);
     // seedOut = M^nPad * seed + tot (the accumulator carried to the next batch)
     void carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot, double* dOut);
@@ -82,8 +87,9 @@ private:
     DevArray<float> prod32_;      // float products of A (mirror)
     DevArray<double> prod64_;     // FP64 products of A
     DevArray<double> A64_, B64_;
-    DevArray<double> Bbar64_;
-    DevArray<float> Bbar32_;
+    DevArray<double> Bbar64_, BbT64_;
+    DevArray<float> Bbar32_, BbT32_;
+    DevArray<float> CT_, DT_;        // C^T and D^T for the output stage
     // power table (transposed), L+1 levels, and FP32 level tables
     DevArray<double> PT64_;
     DevArray<float> PT32_;

@@ -229,8 +229,19 @@ ErrorCode GPUODESolverFixedStepIterative::SimulateODE(const matrixf& A, const ma
                         xVect(jj, ii) = (fType)v;
                         continue;
                     }
+                    // GPUFixedStepIterative.cpp:161-171: float sums, C x then D u
                     fType val = 0.0f;
-                    if (!_bZeroC)                    xVect(jj, ii) = val;
+                    if (!_bZeroC)
+                        for (unsigned kk = 0; kk < _nSystemSize; ++kk) {
+                            fType pr = _hostC(jj, kk) * xVectInit(kk, ii);
+                            val += pr;
+                        }
+                    if (!_bZeroD)
+                        for (unsigned kk = 0; kk < _nInputs; ++kk) {
+                            fType pr = _hostD(jj, kk) * uVectInit(kk, ii);
+                            val += pr;
+                        }
+                    xVect(jj, ii) = val;
                 }
         }
     } catch (std::exception& e) {
@@ -267,7 +278,7 @@ ErrorCode GPUODESolverFixedStepIterative::SimulateODE(const matrixf& A, const ma
                                  _pGPUM->DeviceId(0));
             InitializeBatchData(&matrices, &vectors, 0, tStartBatch, tStep, (int)nb);
             _core->sweep_up(s, nReal, nb);
-            _core->root(s, nb, seedCur, _core->tot());
+            _core->sweep_top(s, nReal, nb, seedCur, _core->tot());
             const bool more = nB + 1 < nBatches;
             if (more) _core->carry(s, nb, seedCur, _core->tot(), seedNext);
             _core->sweep_down(s, nReal, nb);
@@ -382,7 +393,6 @@ void AdamsMoulton::ComputeInitialState(const matrixf& A, const matrixf& B, const
     PARODE_HIP(hipMemcpyAsync(X0.data(), _core->seed(), X0.size() * 8, hipMemcpyDeviceToHost, s));
     _UInit.assign(9, vectorf(m));
     _UInitd.assign(9, std::vector<double>(m, 0.0));
-    _X0d = X0;
     if (m > 0) {
         std::vector<char> raw(9 * (size_t)m * (_bBatchUDouble ? 8 : 4));
         PARODE_HIP(hipMemcpyAsync(raw.data(), Uh, raw.size(), hipMemcpyDeviceToHost, s));
@@ -397,6 +407,7 @@ void AdamsMoulton::ComputeInitialState(const matrixf& A, const matrixf& B, const
             }
     }
     PARODE_HIP(hipStreamSynchronize(s));
+    _X0d = X0;
     x0Scan.resize(4 * (size_t)n);
     for (size_t i = 0; i < X0.size(); ++i) x0Scan[i] = (fType)X0[i];
     (void)x0;
