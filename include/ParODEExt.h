@@ -24,8 +24,10 @@ int ParODE_GetPrecision(void);
  * chain horizons explicitly and for tests of the batch carry. */
 void ParODE_SetMaxBatchSteps(long maxSteps);
 
-/* Kernel family of the FP32 scan levels: 0 = automatic (matrix cores when the
- * scan size allows), 1 = the vector-ALU kernel.  For A/B checks and tuning. */
+/* Kernel choices, for A/B checks and tuning (0 = automatic): bit 0 runs the
+ * FP32 scan levels on the vector ALUs instead of the matrix cores, bit 1 the
+ * FP64 GEMMs of the power table on the vector ALUs, bit 2 launches every level
+ * separately instead of finishing the top of the tree in one work-group. */
 void ParODE_SetLevelKernel(int variant);
 
 /* Compiles a UFunc source for a gfx target (e.g. "gfx950") without a GPU.

@@ -182,8 +182,11 @@ def set_precision(mode):
     lib().ParODE_SetPrecision(mode)
 
 
+VECTOR_LEVELS, VECTOR_GEMM, NO_FUSED_TOP = 1, 2, 4
+
+
 def set_level_kernel(variant):
-    """0 = automatic (matrix cores), 1 = vector-ALU kernel."""
+    """Bit mask (0 = automatic): VECTOR_LEVELS | VECTOR_GEMM | NO_FUSED_TOP."""
     lib().ParODE_SetLevelKernel(int(variant))
 
 

@@ -180,9 +180,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
         }
         __syncthreads();
     }
+    // v_mfma_f64_16x16x4_f64 returns D[i][j] in lane j + 16 (i % 4), register i / 4
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * 16 + 4 * q4 + v, j = j0 + wc * 16 + l16;
+        const int i = i0 + wr * 16 + 4 * v + q4, j = j0 + wc * 16 + l16;
         if (i < r && j < c) Z[(long)i * c + j] = acc[v];
     }
 }
@@ -415,9 +416,12 @@ struct Acc {
     using type = T;
 };
 
+// Threads are grouped per element: 2^tpeLog threads share element e (a power
+// of two >= na, at most 256), so the index arithmetic is shifts and masks.
 template <typename T, typename TU, typename TB>
 __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
-                                                long nPad, const TB* __restrict__ BbT,
+                                                long nPad, int tpeLog,
+                                                const TB* __restrict__ BbT,
                                                 const TU* __restrict__ U, long sL, long oL,
                                                 long rs, T* __restrict__ X)
 {
@@ -426,23 +430,27 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
     // rows r of one element e, so their Bbar reads are contiguous and their
     // input reads are one broadcast.
     using TA = decltype(TB(0) * TU(0));
-    const long total = nPad * na;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
-        const long e = idx / na;
-        const int r = (int)(idx % na) - off;
-        TA v = 0;
-        if (e < nReal && r >= 0 && r < n && m > 0) {
-            for (int q = 0; q < kq; ++q) {
-                const TU* u = U + (e * sL + oL + (long)q * rs) * m;
-                const TB* b = BbT + (long)q * m * n + r;
-                for (int j = 0; j < m; ++j) {
-                    TA pr = (TA)b[(long)j * n] * (TA)u[j];
-                    v = v + pr;
+    const int tpe = 1 << tpeLog;
+    const int epb = 256 >> tpeLog;
+    const int lane = threadIdx.x & (tpe - 1);
+    for (long e = (long)blockIdx.x * epb + (threadIdx.x >> tpeLog); e < nPad;
+         e += (long)gridDim.x * epb) {
+        T* xe = X + e * na;
+        for (int rho = lane; rho < na; rho += tpe) {
+            const int r = rho - off;
+            TA v = 0;
+            if (e < nReal && r >= 0 && r < n && m > 0) {
+                for (int q = 0; q < kq; ++q) {
+                    const TU* u = U + (e * sL + oL + (long)q * rs) * m;
+                    const TB* b = BbT + (long)q * m * n + r;
+                    for (int j = 0; j < m; ++j) {
+                        TA pr = (TA)b[(long)j * n] * (TA)u[j];
+                        v = v + pr;
+                    }
                 }
             }
+            xe[rho] = (T)v;
         }
-        X[idx] = (T)v;
     }
 }
 
@@ -450,9 +458,12 @@ template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, long nPad,
             const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X)
 {
-    long work = nPad * na;
-    hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3(blocks_for(work, 256, 65536)), dim3(256), 0, s,
-                       n, m, kq, na, off, nReal, nPad, Bbar, U, sL, oL, rs, X);
+    int tpeLog = 0;
+    while ((1 << tpeLog) < na && tpeLog < 8) ++tpeLog;
+    const long epb = 256 >> tpeLog;
+    const long blocks = std::min<long>((nPad + epb - 1) / epb, 1L << 16);
+    hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
+                       0, s, n, m, kq, na, off, nReal, nPad, tpeLog, Bbar, U, sL, oL, rs, X);
     check_launch("leaves");
 }
 
@@ -506,7 +517,19 @@ __global__ __launch_bounds__(kLevelThreads) void k_level(int na, long d, long np
 #pragma unroll
         for (int q = 0; q < kCPT; ++q) acc[q] = 0;
         const TP* bcol = Bs + slot;
-        for (int j = 0; j < na; ++j) {
+        int j = 0;
+        for (; j + 4 <= na; j += 4) {
+            TP p4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p4[u] = PT[(long)(j + u) * na + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const TP* brow = bcol + (j + u) * pitch;
+#pragma unroll
+                for (int q = 0; q < kCPT; ++q) acc[q] = fma(p4[u], brow[q * slots], acc[q]);
+            }
+        }
+        for (; j < na; ++j) {
             const TP p = PT[(long)j * na + i];
             const TP* brow = bcol + j * pitch;
 #pragma unroll
@@ -747,7 +770,20 @@ __device__ __forceinline__ void top_update(bool down, int na, long d, long c0, i
         TA acc[kTopMaxCols];
 #pragma unroll
         for (int q = 0; q < kTopMaxCols; ++q) acc[q] = 0;
-        for (int j = 0; j < na; ++j) {
+        int j = 0;
+        for (; j + 8 <= na; j += 8) {
+            TA pj[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pj[u] = P[(long)(j + u) * na + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < kTopMaxCols; ++q) {
+                    const int c = g + q * groups;
+                    if (c < cc) acc[q] = fma(pj[u], (TA)Bsh[c * na + j + u], acc[q]);
+                }
+        }
+        for (; j < na; ++j) {
             const TA pj = P[(long)j * na + i];
 #pragma unroll
             for (int q = 0; q < kTopMaxCols; ++q) {
@@ -883,7 +919,8 @@ void carry(hipStream_t s, int na, const double* P, const double* seed, const dou
 // ----------------------------------------------------------------- outputs
 template <typename TX, typename TU>
 __global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, int oX, bool zC,
-                                                 bool zD, long N, const float* __restrict__ CT,
+                                                 bool zD, long N, int tpeLog,
+                                                 const float* __restrict__ CT,
                                                  const float* __restrict__ DT,
                                                  const TX* __restrict__ X,
                                                  const TU* __restrict__ U, long sO, long oO,
@@ -891,33 +928,36 @@ __global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, in
 {
 #pragma clang fp contract(off)
     // CT = C^T ([n][p]) and DT = D^T ([m][p]): the threads of one element read
-    // one contiguous row of each per term.
+    // one contiguous row of each per term.  2^tpeLog threads per element.
     using TA = decltype(TX(0) * TU(0));
     const int nOut = (zC && zD) ? n : p;
-    const long total = N * nOut;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
-        const long e = idx / nOut;
-        const int j = (int)(idx % nOut);
+    const int tpe = 1 << tpeLog;
+    const int epb = 256 >> tpeLog;
+    const int lane = threadIdx.x & (tpe - 1);
+    for (long e = (long)blockIdx.x * epb + (threadIdx.x >> tpeLog); e < N;
+         e += (long)gridDim.x * epb) {
         const TX* x = X + e * na + (long)oX * n;
-        if (zC && zD) {
-            Y[idx] = (float)x[j];
-            continue;
-        }
-        TA v = 0;
-        if (!zC)
-            for (int c = 0; c < n; ++c) {
-                TA pr = (TA)CT[(long)c * p + j] * (TA)x[c];
-                v = v + pr;
+        float* y = Y + e * nOut;
+        for (int j = lane; j < nOut; j += tpe) {
+            if (zC && zD) {
+                y[j] = (float)x[j];
+                continue;
             }
-        if (!zD && m > 0) {
-            const TU* u = U + (e * sO + oO) * m;
-            for (int c = 0; c < m; ++c) {
-                TA pr = (TA)DT[(long)c * p + j] * (TA)u[c];
-                v = v + pr;
+            TA v = 0;
+            if (!zC)
+                for (int c = 0; c < n; ++c) {
+                    TA pr = (TA)CT[(long)c * p + j] * (TA)x[c];
+                    v = v + pr;
+                }
+            if (!zD && m > 0) {
+                const TU* u = U + (e * sO + oO) * m;
+                for (int c = 0; c < m; ++c) {
+                    TA pr = (TA)DT[(long)c * p + j] * (TA)u[c];
+                    v = v + pr;
+                }
             }
+            y[j] = (float)v;
         }
-        Y[idx] = (float)v;
     }
 }
 
@@ -925,10 +965,14 @@ template <typename TX, typename TU>
 void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
              const float* C, const float* D, const TX* X, const TU* U, long sO, long oO, float* Y)
 {
-    long work = N * ((zC && zD) ? n : p);
-    if (work <= 0) return;
-    hipLaunchKernelGGL((k_outputs<TX, TU>), dim3(blocks_for(work, 256, 65536)), dim3(256), 0, s, n,
-                       m, p, na, oX, zC, zD, N, C, D, X, U, sO, oO, Y);
+    const int nOut = (zC && zD) ? n : p;
+    if (N <= 0 || nOut <= 0) return;
+    int tpeLog = 0;
+    while ((1 << tpeLog) < nOut && tpeLog < 8) ++tpeLog;
+    const long epb = 256 >> tpeLog;
+    const long blocks = std::min<long>((N + epb - 1) / epb, 1L << 16);
+    hipLaunchKernelGGL((k_outputs<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256), 0,
+                       s, n, m, p, na, oX, zC, zD, N, tpeLog, C, D, X, U, sO, oO, Y);
     check_launch("outputs");
 }
 

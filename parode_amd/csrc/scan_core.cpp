@@ -106,7 +106,7 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
             k::abm_combine(s, true, n, m, h, dA, A2, dB, AB, MT, Bbar64_.get(), Bbar32_.get());
         }
     } else {
-        const bool mm = level_kernel() != 1;
+        const bool mm = (level_kernel() & 2) == 0;
         double* A = A64_.get();
         double* B = B64_.get();
         double* A2 = prod64_.get();
@@ -144,7 +144,7 @@ void ScanCore::build_powers(hipStream_t s)
 {
     const size_t na2 = (size_t)na_ * na_;
     // (P^T)^2 = (P^2)^T: squaring the transposed table keeps it transposed
-    const bool mm = level_kernel() != 1;
+    const bool mm = (level_kernel() & 2) == 0;
     for (int l = 0; l < L_; ++l)
         k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
                     PT64_.get() + (l + 1) * na2, mm);
@@ -186,7 +186,7 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
     } else if (level_is_f64(l)) {
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
-    } else if (level_kernel() != 1 && k::level_mfma_supported(na_)) {
+    } else if ((level_kernel() & 1) == 0 && k::level_mfma_supported(na_)) {
         k::level_mfma(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2, X32_.get());
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
@@ -201,7 +201,7 @@ static long pairs_at(long nReal, int l) { return (nReal + (2L << l) - 1) / (2L <
 int ScanCore::top_level(long nReal, long nPad) const
 {
     const int L = ilog2(nPad);
-    if (level_kernel() == 1) return L;               // vector kernels only: no fused top
+    if (level_kernel() & 4) return L;                // fused top switched off
     if ((long)nPad * na_ <= 65536) return 0;         // small: the whole tree in one work-group
     int l = 0;
     while (l < L && pairs_at(nReal, l) > 32) ++l;

@@ -282,7 +282,7 @@ def test_mfma_levels_match_vector_levels(solver, n):
     s = synth.make_system(solver, n, 3, 3, -6.0, -0.2, 1e-3, N, seed=100 + n)
     idx = _register(synth.ufunc_source(s.ucoef))
     outs = []
-    for variant in (1, 0):
+    for variant in (api.VECTOR_LEVELS | api.VECTOR_GEMM | api.NO_FUSED_TOP, 0):
         api.set_level_kernel(variant)
         try:
             tv, y, e = api.simulate_ode(solver, s.A, s.B, s.C, s.D, idx, 0.0, s.t_end, s.h, s.x0)
@@ -296,3 +296,23 @@ def test_mfma_levels_match_vector_levels(solver, n):
     print(f"n_a={s.n_scan}: vector {e_vec:.2e}, mfma {e_mfma:.2e}")
     assert orc.max_rel_err(outs[1], outs[0]) <= PARITY
     assert e_mfma <= PARITY and e_vec <= PARITY
+
+
+@pytest.mark.parametrize("variant", [1, 2, 4, 3, 5, 6])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_each_kernel_choice_meets_parity(solver, variant):
+    """Every combination of the kernel switches (vector/MFMA levels, vector/MFMA
+    FP64 GEMM, fused/unfused top of the tree) meets the parity bar."""
+    s = synth.make_system(solver, 40, 3, 3, -6.0, -0.2, 1e-3, 3000, seed=7 + variant, skew=0.2)
+    idx = _register(synth.ufunc_source(s.ucoef))
+    api.set_level_kernel(variant)
+    try:
+        tv, y, e = api.simulate_ode(solver, s.A, s.B, s.C, s.D, idx, 0.0, s.t_end, s.h, s.x0)
+    finally:
+        api.set_level_kernel(0)
+    assert e == 0, api.last_error_message()
+    ye = orc.simulate_mirror_exact(solver, s.A, s.B, s.C, s.D, s.x0, 0.0, s.t_end, s.h,
+                                   ucoef=s.ucoef)
+    err = orc.max_rel_err(y, ye)
+    print(f"variant {variant}: {err:.2e}")
+    assert err <= PARITY
