@@ -56,8 +56,12 @@ void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, lon
 template <typename TX, typename TP>
 void level(hipStream_t s, bool down, bool delta, int na, long d, long np, const TP* PT, TX* X);
 
-// The same FP32 level update on the matrix cores (17 <= na <= 1024).
+// The same FP32 level update on the matrix cores (17 <= na <= 1024), reading a
+// zero-padded table [mfma_table_rows(na)][32*ceil(na/32)].
 bool level_mfma_supported(int na);
+int mfma_table_rows(int na);
+void make_level_table_padded(hipStream_t s, int na, int nap, int nrp, const double* P, bool delta,
+                             float* PTp);
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
                 float* X);
 

@@ -394,6 +394,33 @@ __global__ void k_level_table(long nn, int na, const double* __restrict__ PT64, 
     }
 }
 
+// Zero-padded FP32 table for the MFMA levels: PTp[j][i] for j < nrp, i < nap.
+__global__ void k_level_table_padded(int na, int nap, int nrp, const double* __restrict__ PT64,
+                                     bool delta, float* __restrict__ PTp)
+{
+    const long total = (long)nrp * nap;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(idx / nap), i = (int)(idx % nap);
+        float v = 0.0f;
+        if (j < na && i < na) {
+            double x = PT64[(long)j * na + i];
+            if (delta && i == j) x -= 1.0;
+            v = (float)x;
+        }
+        PTp[idx] = v;
+    }
+}
+
+void make_level_table_padded(hipStream_t s, int na, int nap, int nrp, const double* P, bool delta,
+                             float* PTp)
+{
+    const long work = (long)nrp * nap;
+    hipLaunchKernelGGL(k_level_table_padded, dim3(blocks_for(work, 256, 8192)), dim3(256), 0, s, na,
+                       nap, nrp, P, delta, PTp);
+    check_launch("level_table_padded");
+}
+
 void make_level_table_f32(hipStream_t s, int na, const double* P, bool delta, float* PT)
 {
     long nn = (long)na * na;
@@ -416,8 +443,12 @@ struct Acc {
     using type = T;
 };
 
-// Threads are grouped per element: 2^tpeLog threads share element e (a power
-// of two >= na, at most 256), so the index arithmetic is shifts and masks.
+// A work-group takes EB consecutive elements per thread group: the inputs of
+// its elements are staged in LDS, and each thread forms one row r of all EB
+// leaves, so every Bbar value it loads is used EB times.  Threads are grouped
+// by 2^tpeLog (>= na, <= 256) per element block; index math is shifts.
+constexpr int kLeafEB = 8;
+
 template <typename T, typename TU, typename TB>
 __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
                                                 long nPad, int tpeLog,
@@ -426,31 +457,44 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
                                                 long rs, T* __restrict__ X)
 {
 #pragma clang fp contract(off)
-    // BbT is Bbar transposed, [kq*m][n]: consecutive threads take consecutive
-    // rows r of one element e, so their Bbar reads are contiguous and their
-    // input reads are one broadcast.
     using TA = decltype(TB(0) * TU(0));
+    extern __shared__ unsigned char leaf_smem[];
+    TA* Us = reinterpret_cast<TA*>(leaf_smem);       // [groups*EB][kq*m]
     const int tpe = 1 << tpeLog;
-    const int epb = 256 >> tpeLog;
-    const int lane = threadIdx.x & (tpe - 1);
-    for (long e = (long)blockIdx.x * epb + (threadIdx.x >> tpeLog); e < nPad;
-         e += (long)gridDim.x * epb) {
-        T* xe = X + e * na;
+    const int groups = 256 >> tpeLog;
+    const int grp = threadIdx.x >> tpeLog, lane = threadIdx.x & (tpe - 1);
+    const int km = kq * m;
+    const long perBlock = (long)groups * kLeafEB;
+    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
+        // stage the inputs of the block's elements
+        for (int t = threadIdx.x; t < perBlock * km; t += 256) {
+            const int le = t / km, c = t % km, q = c / (m > 0 ? m : 1), j = c % (m > 0 ? m : 1);
+            const long e = eb + le;
+            Us[t] = (e < nReal && m > 0) ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
+        }
+        __syncthreads();
+        const long e0 = eb + (long)grp * kLeafEB;
+        const TA* ug = Us + (long)grp * kLeafEB * km;
         for (int rho = lane; rho < na; rho += tpe) {
             const int r = rho - off;
-            TA v = 0;
-            if (e < nReal && r >= 0 && r < n && m > 0) {
-                for (int q = 0; q < kq; ++q) {
-                    const TU* u = U + (e * sL + oL + (long)q * rs) * m;
-                    const TB* b = BbT + (long)q * m * n + r;
-                    for (int j = 0; j < m; ++j) {
-                        TA pr = (TA)b[(long)j * n] * (TA)u[j];
-                        v = v + pr;
+            TA v[kLeafEB];
+#pragma unroll
+            for (int u = 0; u < kLeafEB; ++u) v[u] = 0;
+            if (r >= 0 && r < n) {
+                for (int c = 0; c < km; ++c) {
+                    const TA b = (TA)BbT[(long)c * n + r];
+#pragma unroll
+                    for (int u = 0; u < kLeafEB; ++u) {
+                        TA pr = b * ug[u * km + c];
+                        v[u] = v[u] + pr;
                     }
                 }
             }
-            xe[rho] = (T)v;
+#pragma unroll
+            for (int u = 0; u < kLeafEB; ++u)
+                if (e0 + u < nPad) X[(e0 + u) * na + rho] = (T)v[u];
         }
+        __syncthreads();
     }
 }
 
@@ -458,12 +502,14 @@ template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, long nPad,
             const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X)
 {
+    using TA = decltype(TB(0) * TU(0));
     int tpeLog = 0;
     while ((1 << tpeLog) < na && tpeLog < 8) ++tpeLog;
-    const long epb = 256 >> tpeLog;
-    const long blocks = std::min<long>((nPad + epb - 1) / epb, 1L << 16);
+    const long perBlock = (256L >> tpeLog) * kLeafEB;
+    const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
+    const size_t lds = (size_t)perBlock * kq * (m > 0 ? m : 1) * sizeof(TA);
     hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
-                       0, s, n, m, kq, na, off, nReal, nPad, tpeLog, Bbar, U, sL, oL, rs, X);
+                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, Bbar, U, sL, oL, rs, X);
     check_launch("leaves");
 }
 
@@ -616,14 +662,18 @@ template void level<double, double>(hipStream_t, bool, bool, int, long, long, co
 // time with an odd pitch (conflict-free column reads).
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// PT is the zero-padded transposed table, [nrp][nap] with nap = 32*ceil(na/32)
+// and nrp a multiple of KC, so fragment loads need no bounds checks; the next
+// k-step's B fragments are loaded while the current MFMAs run.
 template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC>
-__global__ __launch_bounds__(256) void k_level_mfma(int na, long d, long np,
+__global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, long d, long np,
                                                     const float* __restrict__ PT,
                                                     float* __restrict__ X)
 {
     constexpr int WGC = 4 / WGI;
     constexpr int TC = WGC * NCT * 32;
     constexpr int PITCH = KC + 1;
+    constexpr int PER = TC * KC / 256;
     __shared__ float As[TC * PITCH];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wi = wave % WGI, wc = wave / WGI;
@@ -637,65 +687,102 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, long d, long np,
         for (int b = 0; b < NIT; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    const float* pcol[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) pcol[it] = PT + (wi + it * WGI) * 32 + l32;
     for (int j0 = 0; j0 < na; j0 += KC) {
         const int kc = (na - j0) < KC ? (na - j0) : KC;
-        for (int t = threadIdx.x; t < TC * KC; t += 256) {
+        float v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = threadIdx.x + u * 256;
             const int c = t / KC, j = t % KC;
-            float v = 0.0f;
-            if (c < ncol && j < kc) {
-                const long R = (k0 + c + 1) * 2 * d - 1;
-                const long src = DOWN ? R : R - d;
-                v = X[src * na + j0 + j];
-            }
-            As[c * PITCH + j] = v;
+            const long R = (k0 + c + 1) * 2 * d - 1;
+            v[u] = (c < ncol && j < kc) ? X[(DOWN ? R : R - d) * na + j0 + j] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int t = threadIdx.x + u * 256;
+            As[(t / KC) * PITCH + (t % KC)] = v[u];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < KC; kk += 2) {
-            const int kb = j0 + kk + hi;
-            float bf[NIT];
+        // k-steps of this slice: up to the 32-padded width (the padded table is
+        // zero there and the staged columns are zero beyond na).  B fragments
+        // are fetched KU k-steps at a time, one block ahead of the MFMAs that
+        // use them; rows past the table are clamped (loaded, never used).
+        constexpr int KU = 8;
+        const int kend = (nap - j0) < KC ? (nap - j0) : KC;   // a multiple of 32
+        float bc[KU][NIT], bn[KU][NIT];
 #pragma unroll
-            for (int it = 0; it < NIT; ++it) {
-                const int i = (wi + it * WGI) * 32 + l32;
-                bf[it] = (kb < na && i < na) ? PT[(long)kb * na + i] : 0.0f;
+        for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) bc[u][it] = pcol[it][(long)(j0 + 2 * u + hi) * nap];
+        for (int kb = 0; kb < kend; kb += 2 * KU) {
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                int row = j0 + kb + 2 * KU + 2 * u + hi;
+                row = row < nrp ? row : nrp - 1;
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) bn[u][it] = pcol[it][(long)row * nap];
             }
-            float af[NCT];
+            // keep the prefetch ahead of this block's MFMAs
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
-                af[ct] = As[((wc + ct * WGC) * 32 + l32) * PITCH + kk + hi];
+            for (int u = 0; u < KU; ++u) {
+                const int kk = kb + 2 * u;
+                float af[NCT];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
+                for (int ct = 0; ct < NCT; ++ct)
+                    af[ct] = As[((wc + ct * WGC) * 32 + l32) * PITCH + kk + hi];
 #pragma unroll
-                for (int it = 0; it < NIT; ++it)
-                    acc[ct][it] =
-                        __builtin_amdgcn_mfma_f32_32x32x2f32(af[ct], bf[it], acc[ct][it], 0, 0, 0);
+                for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                    for (int it = 0; it < NIT; ++it)
+                        acc[ct][it] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[ct], bc[u][it],
+                                                                           acc[ct][it], 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < KU; ++u)
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) bc[u][it] = bn[u][it];
         }
         __syncthreads();
     }
+    const bool full = ncol == TC;
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int i = (wi + it * WGI) * 32 + l32;
             if (i >= na) continue;
+            float xr[16], xl[16];
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                if (cr >= ncol) continue;
-                const long R = (k0 + cr + 1) * 2 * d - 1, L = R - d;
+                const long R = (k0 + cr + 1) * 2 * d - 1;
+                const bool ok = full || cr < ncol;
+                xr[v] = ok ? X[R * na + i] : 0.0f;
+                xl[v] = (ok && (DOWN || DELTA)) ? X[(R - d) * na + i] : 0.0f;
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                if (!full && cr >= ncol) continue;
+                const long R = (k0 + cr + 1) * 2 * d - 1;
                 float y = acc[ct][it][v];
                 if (!DOWN) {
-                    if (DELTA) y = y + X[L * na + i];
-                    X[R * na + i] = X[R * na + i] + y;
+                    if (DELTA) y = y + xl[v];
+                    X[R * na + i] = xr[v] + y;
                 } else {
-                    const float xr = X[R * na + i], xl = X[L * na + i];
-                    if (DELTA) y = y + xr;
-                    X[L * na + i] = xr;
-                    X[R * na + i] = y + xl;
+                    if (DELTA) y = y + xr[v];
+                    X[(R - d) * na + i] = xr[v];
+                    X[R * na + i] = y + xl[v];
                 }
             }
         }
 }
+
+int mfma_table_rows(int na);
 
 template <int NIT, int NCT, int WGI>
 static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
@@ -703,24 +790,28 @@ static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, lo
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
     constexpr int KC = 8192 / TC;
+    const int nap = (na + 31) / 32 * 32, nrp = mfma_table_rows(na);
     dim3 g((unsigned)((np + TC - 1) / TC)), b(256);
     if (down) {
         if (delta)
-            hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
-                               PT, X);
+            hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap, nrp, d,
+                               np, PT, X);
         else
-            hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
-                               PT, X);
+            hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
+                               nrp, d, np, PT, X);
     } else {
         if (delta)
-            hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, d, np,
-                               PT, X);
+            hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
+                               nrp, d, np, PT, X);
         else
-            hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, d,
-                               np, PT, X);
+            hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
+                               nrp, d, np, PT, X);
     }
     check_launch("level_mfma");
 }
+
+// Rows of the padded MFMA table: a multiple of every K-slice (<= 256).
+int mfma_table_rows(int na) { return (na + 255) / 256 * 256; }
 
 bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
 
@@ -758,7 +849,7 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
 // with only barriers between levels.  Levels >= l64 use the FP64 table.
 // Thread t owns state i = t mod rowThreads and the column group t / rowThreads.
 constexpr int kTopThreads = 1024;
-constexpr int kTopMaxCols = 8;
+constexpr int kTopMaxCols = 4;
 
 // One level's update of cc staged columns for the rows of thread (i0, g).
 template <typename TX, typename TA>
@@ -771,12 +862,12 @@ __device__ __forceinline__ void top_update(bool down, int na, long d, long c0, i
 #pragma unroll
         for (int q = 0; q < kTopMaxCols; ++q) acc[q] = 0;
         int j = 0;
-        for (; j + 8 <= na; j += 8) {
-            TA pj[8];
+        for (; j + 4 <= na; j += 4) {
+            TA pj[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) pj[u] = P[(long)(j + u) * na + i];
+            for (int u = 0; u < 4; ++u) pj[u] = P[(long)(j + u) * na + i];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int q = 0; q < kTopMaxCols; ++q) {
                     const int c = g + q * groups;
@@ -802,6 +893,66 @@ __device__ __forceinline__ void top_update(bool down, int na, long d, long c0, i
                 const TA xl = (TA)X[Lp * na + i];
                 X[Lp * na + i] = (TX)Bsh[c * na + i];
                 X[R * na + i] = (TX)(acc[q] + xl);
+            }
+        }
+    }
+}
+
+// Update for na >= 64: lanes r16 + 16 kq of a wave share row i = 16 w + r16
+// and split the j range in four (kq), so each thread's chain of table loads
+// is a quarter as long; the four partial sums meet by cross-lane shuffles.
+constexpr int kTopKsCols = 16;
+
+template <typename TX, typename TA>
+__device__ __forceinline__ void top_update_ksplit(bool down, int na, long d, long c0, int cc,
+                                                  const TA* __restrict__ P, const double* Bsh,
+                                                  TX* X)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int jq = (na + 3) / 4;
+    const int jlo = kq * jq, jhi = (jlo + jq) < na ? jlo + jq : na;
+    for (int rp = 0; rp < na; rp += 256) {
+        const int i = rp + w * 16 + r16;
+        const bool live = i < na;
+        const int ic = live ? i : 0;
+        TA acc[kTopKsCols];
+#pragma unroll
+        for (int c = 0; c < kTopKsCols; ++c) acc[c] = 0;
+        int j = jlo;
+        for (; j + 4 <= jhi; j += 4) {
+            TA pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = P[(long)(j + u) * na + ic];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int c = 0; c < kTopKsCols; ++c)
+                    if (c < cc) acc[c] = fma(pj[u], (TA)Bsh[c * na + j + u], acc[c]);
+        }
+        for (; j < jhi; ++j) {
+            const TA pj = P[(long)j * na + ic];
+#pragma unroll
+            for (int c = 0; c < kTopKsCols; ++c)
+                if (c < cc) acc[c] = fma(pj, (TA)Bsh[c * na + j], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < kTopKsCols; ++c) {
+            acc[c] += __shfl_xor(acc[c], 16);
+            acc[c] += __shfl_xor(acc[c], 32);
+        }
+        if (live && kq == 0) {
+#pragma unroll
+            for (int c = 0; c < kTopKsCols; ++c) {
+                if (c >= cc) continue;
+                const long R = (c0 + c + 1) * 2 * d - 1, Lp = R - d;
+                if (!down) {
+                    X[R * na + i] = (TX)((TA)X[R * na + i] + acc[c]);
+                } else {
+                    const TA xl = (TA)X[Lp * na + i];
+                    X[Lp * na + i] = (TX)Bsh[c * na + i];
+                    X[R * na + i] = (TX)(acc[c] + xl);
+                }
             }
         }
     }
@@ -835,7 +986,14 @@ __global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int 
                     Bsh[c * na + j] = (double)X[(down ? R : R - d) * na + j];
                 }
                 __syncthreads();
-                if (g < groups) {
+                if (na >= 64) {
+                    if (l >= l64)
+                        top_update_ksplit<TX, double>(down, na, d, c0, cc, PT64 + (long)l * na2,
+                                                      Bsh, X);
+                    else
+                        top_update_ksplit<TX, float>(down, na, d, c0, cc, PT32 + (long)l * na2,
+                                                     Bsh, X);
+                } else if (g < groups) {
                     if (l >= l64)
                         top_update<TX, double>(down, na, d, c0, cc, i0, rowThreads, g, groups,
                                                PT64 + (long)l * na2, Bsh, X);
@@ -865,7 +1023,7 @@ void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float*
     // what 48 KiB of LDS can stage
     const int rowThreads = na < kTopThreads ? na : kTopThreads;
     const int groups = kTopThreads / rowThreads;
-    int CC = std::min(groups * kTopMaxCols, 6144 / na);
+    int CC = na >= 64 ? std::min(kTopKsCols, 6144 / na) : std::min(groups * kTopMaxCols, 6144 / na);
     if (CC < 1) CC = 1;
     size_t lds = (size_t)CC * na * sizeof(double);
     hipLaunchKernelGGL((k_top<TX>), dim3(1), dim3(kTopThreads), lds, s, na, l0, L, l64, nReal,

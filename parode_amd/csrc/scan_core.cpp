@@ -31,6 +31,11 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     for (int l = 0; l < L_; ++l)
         if (!level_is_f64(l)) n32 = l + 1;
     if (n32 > 0) PT32_.alloc(na2 * n32);
+    if (n32 > 0 && k::level_mfma_supported(na_)) {
+        nap_ = (na_ + 31) / 32 * 32;
+        nrp_ = k::mfma_table_rows(na_);
+        PTm_.alloc((size_t)nap_ * nrp_ * n32);
+    }
     if (xIsDouble())
         X64_.alloc((size_t)nPadMax_ * na_);
     else
@@ -51,6 +56,8 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     b += 24 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
     b += 8 * na * na * (L + 1);
     if (opt.precision != kFp64) b += 4 * na * na * L;
+    if (opt.precision != kFp64 && na >= 17 && na <= 1024)
+        b += 4 * (size_t)((na + 31) / 32 * 32) * ((na + 255) / 256 * 256) * L;
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
     b += 24 * na;
     return b;
@@ -60,7 +67,7 @@ size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
            Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + CT_.bytes() + DT_.bytes() +
-           PT64_.bytes() + PT32_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
+           PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
            tot_.bytes() + seedNext_.bytes();
 }
 
@@ -149,9 +156,14 @@ void ScanCore::build_powers(hipStream_t s)
         k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
                     PT64_.get() + (l + 1) * na2, mm);
     for (int l = 0; l < L_; ++l)
-        if (!level_is_f64(l))
+        if (!level_is_f64(l)) {
             k::make_level_table_f32(s, na_, PT64_.get() + l * na2, level_is_delta(l),
                                     PT32_.get() + l * na2);
+            if (PTm_.get())
+                k::make_level_table_padded(s, na_, nap_, nrp_, PT64_.get() + l * na2,
+                                           level_is_delta(l),
+                                           PTm_.get() + (size_t)l * nap_ * nrp_);
+        }
 }
 
 // -------------------------------------------------------------------- leaves
@@ -186,8 +198,9 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
     } else if (level_is_f64(l)) {
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
-    } else if ((level_kernel() & 1) == 0 && k::level_mfma_supported(na_)) {
-        k::level_mfma(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2, X32_.get());
+    } else if ((level_kernel() & 1) == 0 && PTm_.get()) {
+        k::level_mfma(s, down, level_is_delta(l), na_, d, np,
+                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get());
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
                                X32_.get());

@@ -93,6 +93,8 @@ private:
     // power table (transposed), L+1 levels, and FP32 level tables
     DevArray<double> PT64_;
     DevArray<float> PT32_;
+    DevArray<float> PTm_;            // zero-padded FP32 tables of the MFMA levels
+    int nap_ = 0, nrp_ = 0;
     // trajectory
     DevArray<float> X32_;
     DevArray<double> X64_;

@@ -48,8 +48,12 @@ def test_plan_equals_c_abi_sampled_path(solver):
     y = _run_plan(s, U, api.PLAIN32)
     tv, yc, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h, s.x0)
     assert e == 0
-    # same kernels, same order: identical results
-    np.testing.assert_array_equal(y, yc)
+    # same scan kernels in the same order: identical scan outputs; the ABM
+    # start-up outputs are formed on the device (FP64 sums) by the plan and on
+    # the host (float sums, GPUFixedStepIterative.cpp:161-171) by the C ABI path
+    lead = 3 if solver == 1 else 0
+    np.testing.assert_array_equal(y[lead:], yc[lead:])
+    assert orc.max_rel_err(y[:lead], yc[:lead]) <= 1e-6 if lead else True
     api.close_gpu()
 
 
