@@ -48,7 +48,13 @@
  *   or_simulate_*        reference scenarios reproduced with glibc rand(),
  *                        passing the reference's own sqrt(sum err^2) < 0.1 rule
  *                        against or_cpu_rk4_odeint / or_cpu_abm_pece
+ *   or_simulate_mirror_exact  agrees with or_simulate_mirror (FP32 scan) and
+ *                        or_simulate_f64 within their rounding at short
+ *                        horizons, and equals the FP64 recurrence driven by the
+ *                        mirror's float M, Bbar and samples
  *   or_gauss_noise       moments of the generated samples
+ *   all modes            golden fixtures (tests/golden/*.npz) guard against
+ *                        regressions once the checks above pass
  * What no check can pin (parity unpinned): the FP32 summation order and FMA
  * contraction inside the reference's OpenCL kernels, the OpenCL runtime's sin/cos
  * ulps, and Boost.uBLAS / Boost.Odeint internals beyond their documented

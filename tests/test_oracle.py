@@ -294,3 +294,27 @@ def test_reference_c_abi_scenario(solver, steps):
     yc = cpu(d["A"], d["B"], d["C"], d["D"], d["x0"], Uh, total, h)
     lo = 1 if solver == 0 else 4
     assert _l2(y[lo:total - 3], yc[lo:total - 3]) < 0.1        # TestParODEStaticLibCInterface.cpp:204,466
+
+
+def test_mirror_exact_is_fp64_recurrence_on_mirror_inputs():
+    """or_simulate_mirror_exact (RK4) = an FP64 numpy recurrence driven by the
+    mirror's float M and Bbar and the float samples: x_{k+1} = M x_k +
+    Bbar [u_2k; u_2k+1; u_2k+2], y_k = C x_k + D u_2k."""
+    rng = np.random.default_rng(7)
+    n, m, p, N, h = 5, 2, 3, 40, 0.05
+    A = (rng.standard_normal((n, n)) * 0.5 - 2.0 * np.eye(n)).astype(np.float32)
+    B = rng.standard_normal((n, m)).astype(np.float32)
+    C = rng.standard_normal((p, n)).astype(np.float32)
+    D = rng.standard_normal((p, m)).astype(np.float32)
+    x0 = rng.standard_normal(n).astype(np.float32)
+    U = rng.standard_normal((2 * N + 1, m)).astype(np.float32)
+    y = orc.simulate_mirror_exact(0, A, B, C, D, x0, 0.0, (N - 1) * h + 1e-4 * h, h, Uhalf=U)
+    M, Bb = orc.rk4_build_mirror(A, B, h)
+    M, Bb, U64 = M.astype(np.float64), Bb.astype(np.float64), U.astype(np.float64)
+    x = x0.astype(np.float64)
+    ref = np.empty((N, p))
+    for k in range(N):
+        ref[k] = C.astype(np.float64) @ x + D.astype(np.float64) @ U64[2 * k]
+        v = Bb[:, :m] @ U64[2 * k] + Bb[:, m:2 * m] @ U64[2 * k + 1] + Bb[:, 2 * m:] @ U64[2 * k + 2]
+        x = M @ x + v
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
