@@ -53,7 +53,7 @@
  *                        horizons, and equals the FP64 recurrence driven by the
  *                        mirror's float M, Bbar and samples
  *   or_gauss_noise       moments of the generated samples
- *   all modes            golden fixtures (tests/golden/*.npz) guard against
+ *   all modes            golden fixtures (the .npz files of tests/golden) guard against
  *                        regressions once the checks above pass
  * What no check can pin (parity unpinned): the FP32 summation order and FMA
  * contraction inside the reference's OpenCL kernels, the OpenCL runtime's sin/cos
