@@ -78,6 +78,40 @@ int ParODE_PlanDestroy(void* plan);
 /* Device memory the plan for this shape would hold, without allocating. */
 long ParODE_PlanEstimateBytes(int solver, int nStates, int nInputs, long points, int precision);
 
+/* ---- serving: continuous batching of independent solves on one GPU ----
+ * For request loops with many systems of different shapes and horizons.  A
+ * server owns a device arena (deviceBytes), a pinned staging area
+ * (pinnedBytes), `lanes` HIP streams (1..16) and a worker thread.  Submitted
+ * requests are admitted in arrival order whenever a lane has room and the
+ * arena holds the request's footprint (ParODE_ServerFootprint); an admitted
+ * request's whole solve runs on its lane with no host synchronisation, so
+ * requests overlap on the device and arrivals are admitted while others run.
+ * The server copies A, B, C, D, x0 and coef at submission; samples (host,
+ * (2*points+1) x nInputs doubles on the h/2 grid from tStart) and Y (host,
+ * points x nOut floats) must stay valid until the ticket completes.  With
+ * samples == NULL the inputs are the family u_j = a sin(w t + phi) +
+ * b cos(w t + phi) + c + d t (+ sigma N(0,1) keyed by (seed, sample, j)),
+ * coef row j = (a, b, c, d, w, phi), sampled in FP64 on the device.
+ * precision as ParODE_SetPrecision (mixed = 1 is the serving default). */
+int ParODE_ServerCreate(int lanes, long deviceBytes, long pinnedBytes, int precision,
+						void** server);
+int ParODE_ServerSubmit(void* server, int solver, int nStates, int nInputs, int nOutputs,
+						int zeroC, int zeroD, long points, double tStart, double tStep,
+						const float* A, const float* B, const float* C, const float* D,
+						const float* x0, const double* coef, double sigma,
+						unsigned long long seed, const double* samples, float* Y,
+						long* ticket);
+/* Blocks until the ticket completes; returns its status.  timesMs (may be
+ * NULL) receives [submission -> completion, submission -> admission]. */
+int ParODE_ServerWait(void* server, long ticket, double* timesMs);
+int ParODE_ServerDrain(void* server);
+/* stats[8]: submitted, completed, failed, running, queued, peak running,
+ * arena bytes, peak arena bytes in use. */
+int ParODE_ServerStats(void* server, long* stats);
+long ParODE_ServerFootprint(int solver, int nStates, int nInputs, int nOutputs, long points,
+							int precision);
+int ParODE_ServerDestroy(void* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,10 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_CompileUFunc", "ParODE_GetLastErrorMessage", "SimulateODESampled",
                "ParODE_PlanCreate", "ParODE_PlanExecute", "ParODE_PlanExecuteTimed",
                "ParODE_PlanFootprint",
-               "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes"]
+               "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes",
+               "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
+               "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
+               "ParODE_ServerDestroy"]
 
 RK4, ABM = 0, 1
 PLAIN32, MIXED, FP64 = 0, 1, 2
@@ -91,6 +94,23 @@ def lib():
         L.ParODE_PlanEstimateBytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_long, ctypes.c_int]
         L.ParODE_PlanEstimateBytes.restype = ctypes.c_long
+        _l, _i32, _d = ctypes.c_long, ctypes.c_int, ctypes.c_double
+        L.ParODE_ServerCreate.argtypes = [_i32, _l, _l, _i32, ctypes.POINTER(_vp)]
+        L.ParODE_ServerCreate.restype = _i32
+        L.ParODE_ServerSubmit.argtypes = ([_vp] + [_i32] * 6 + [_l, _d, _d] + [_fp] * 5 +
+                                          [_dp, _d, ctypes.c_ulonglong, _dp, _fp,
+                                           ctypes.POINTER(_l)])
+        L.ParODE_ServerSubmit.restype = _i32
+        L.ParODE_ServerWait.argtypes = [_vp, _l, _dp]
+        L.ParODE_ServerWait.restype = _i32
+        L.ParODE_ServerDrain.argtypes = [_vp]
+        L.ParODE_ServerDrain.restype = _i32
+        L.ParODE_ServerStats.argtypes = [_vp, ctypes.POINTER(_l)]
+        L.ParODE_ServerStats.restype = _i32
+        L.ParODE_ServerFootprint.argtypes = [_i32] * 4 + [_l, _i32]
+        L.ParODE_ServerFootprint.restype = _l
+        L.ParODE_ServerDestroy.argtypes = [_vp]
+        L.ParODE_ServerDestroy.restype = _i32
         _lib = L
     return _lib
 
@@ -299,6 +319,78 @@ class Plan:
         if self.handle:
             lib().ParODE_PlanDestroy(self.handle)
             self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (ParODE_Server*): submit() returns a
+    ticket at once; the library's worker thread admits requests onto its lanes
+    as device memory allows; wait() returns (status, latency_ms, queue_ms) and
+    the outputs are in the array given to submit()."""
+
+    STATS = ("submitted", "completed", "failed", "running", "queued", "peak_running",
+             "arena_bytes", "arena_peak")
+
+    def __init__(self, lanes=4, device_bytes=16 << 30, pinned_bytes=1 << 30, precision=MIXED):
+        self.handle = ctypes.c_void_p()
+        r = lib().ParODE_ServerCreate(lanes, device_bytes, pinned_bytes, precision,
+                                      ctypes.byref(self.handle))
+        if r != 0:
+            raise ParODEError(r, "ParODE_ServerCreate")
+        self._keep = {}
+
+    def submit(self, solver, A, B, C, D, x0, points, h, t0=0.0, coef=None, sigma=0.0, seed=0,
+               samples=None, Y=None):
+        """Queues one solve; returns (ticket, Y) where Y (points x nOut float32)
+        is filled when the ticket completes."""
+        A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
+        n_out = n if (zC and zD) else p
+        if Y is None:
+            Y = np.zeros((points, n_out), np.float32)
+        assert Y.dtype == np.float32 and Y.flags.c_contiguous and Y.size >= points * n_out
+        cf = None if coef is None else np.ascontiguousarray(coef, np.float64)
+        sm = None if samples is None else np.ascontiguousarray(samples, np.float64)
+        t = ctypes.c_long(0)
+        r = lib().ParODE_ServerSubmit(
+            self.handle, solver, n, m, p, zC, zD, points, t0, h, _fptr(A), _fptr(Bf), _fptr(Cf),
+            _fptr(Df), _fptr(x0f),
+            None if cf is None else cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            sigma, seed,
+            None if sm is None else sm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            Y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(t))
+        if r != 0:
+            raise ParODEError(r, "ParODE_ServerSubmit")
+        self._keep[t.value] = (sm, Y)
+        return t.value, Y
+
+    def wait(self, ticket):
+        ms = (ctypes.c_double * 2)()
+        st = lib().ParODE_ServerWait(self.handle, ticket, ms)
+        self._keep.pop(ticket, None)
+        return st, ms[0], ms[1]
+
+    def drain(self):
+        lib().ParODE_ServerDrain(self.handle)
+
+    def stats(self):
+        v = (ctypes.c_long * 8)()
+        lib().ParODE_ServerStats(self.handle, v)
+        return dict(zip(self.STATS, list(v)))
+
+    @staticmethod
+    def footprint(solver, n, m, p, points, precision=MIXED):
+        return lib().ParODE_ServerFootprint(solver, n, m, p, points, precision)
+
+    def close(self):
+        if self.handle:
+            lib().ParODE_ServerDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+            self._keep.clear()
 
     def __del__(self):
         try:

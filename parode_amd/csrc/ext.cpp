@@ -36,33 +36,8 @@ enum { kPhaseMap = 0, kPhaseLeaves, kPhaseUp, kPhaseRoot, kPhaseDown, kPhaseOutp
 void run_plan(Plan* pl, const float* dA, const float* dB, const float* dC, const float* dD,
               const float* dx0, const void* dSamples, float* dY, hipStream_t s, bool mark)
 {
-    auto tick = [&](int i) {
-        if (mark) PARODE_HIP(hipEventRecord(pl->ev[i], s));
-    };
-    parode::ScanCore& c = *pl->core;
-    tick(kPhaseMap);
-    c.build_map(s, dA, dB, pl->h);
-    c.build_powers(s);
-    tick(kPhaseLeaves);
-    float* yScan = dY;
-    if (pl->solver == 0) {
-        parode::k::to_f64<float>(s, pl->n, dx0, c.seed());
-        c.leaves(s, dSamples, pl->uDouble, 2, 0, 1, pl->nScan, pl->nPad);
-    } else {
-        c.abm_startup(s, dA, dB, dx0, dSamples, pl->uDouble, pl->h, c.seed());
-        c.startup_outputs(s, dC, dD, c.seed(), dSamples, pl->uDouble, dY);
-        yScan = dY + 3L * c.nOut();
-        c.leaves(s, dSamples, pl->uDouble, 2, 0, 2, pl->nScan, pl->nPad);
-    }
-    tick(kPhaseUp);
-    c.sweep_up(s, pl->nScan, pl->nPad);
-    tick(kPhaseRoot);
-    c.sweep_top(s, pl->nScan, pl->nPad, c.seed(), c.tot());
-    tick(kPhaseDown);
-    c.sweep_down(s, pl->nScan, pl->nPad);
-    tick(kPhaseOutputs);
-    c.outputs(s, dC, dD, dSamples, pl->uDouble, 2, pl->solver == 0 ? 0 : 6, pl->nScan, yScan);
-    tick(kPhases);
+    pl->core->solve(s, pl->points, pl->h, dA, dB, dC, dD, dx0, dSamples, pl->uDouble, dY,
+                    mark ? pl->ev : nullptr);
 }
 
 int shape_error(int solver, int n, int m, long points)

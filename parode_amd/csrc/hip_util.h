@@ -3,8 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <iterator>
+#include <map>
+#include <mutex>
+#include <new>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 
 namespace parode {
 
@@ -21,38 +26,147 @@ inline void hip_check(hipError_t e, const char* what)
 
 #define PARODE_HIP(call) ::parode::hip_check((call), #call)
 
-// Device allocation owned by its holder; moves, never copies.
+// First-fit sub-allocator over one region (device or pinned host memory),
+// 256-byte aligned blocks, neighbours coalesced on release.  Thread-safe.
+class Arena {
+public:
+    static constexpr size_t kAlign = 256;
+    enum Kind { kDevice, kPinnedHost };
+    Arena(size_t bytes, Kind kind) : kind_(kind)
+    {
+        cap_ = round(bytes);
+        if (cap_ == 0) return;
+        if (kind == kDevice)
+            PARODE_HIP(hipMalloc(reinterpret_cast<void**>(&base_), cap_));
+        else
+            PARODE_HIP(hipHostMalloc(reinterpret_cast<void**>(&base_), cap_, hipHostMallocDefault));
+        free_[0] = cap_;
+    }
+    ~Arena()
+    {
+        if (!base_) return;
+        if (kind_ == kDevice)
+            (void)hipFree(base_);
+        else
+            (void)hipHostFree(base_);
+    }
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+
+    static size_t round(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+    // a block of at least `bytes`, or nullptr when no free range fits
+    void* take(size_t bytes)
+    {
+        const size_t b = round(bytes ? bytes : 1);
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < b) continue;
+            const size_t off = it->first, rest = it->second - b;
+            free_.erase(it);
+            if (rest) free_[off + b] = rest;
+            live_[off] = b;
+            used_ += b;
+            return base_ + off;
+        }
+        return nullptr;
+    }
+    void give(void* p)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu_);
+        const size_t off = static_cast<char*>(p) - base_;
+        auto lv = live_.find(off);
+        if (lv == live_.end()) return;
+        size_t start = off, size = lv->second;
+        used_ -= size;
+        live_.erase(lv);
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && next->first == start + size) {
+            size += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                start = prev->first;
+                size += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[start] = size;
+    }
+    size_t capacity() const { return cap_; }
+    size_t in_use()
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        return used_;
+    }
+    // the largest block take() could return now
+    size_t largest_free()
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        size_t best = 0;
+        for (auto& f : free_) best = f.second > best ? f.second : best;
+        return best;
+    }
+
+private:
+    Kind kind_;
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;             // offset -> size
+    std::unordered_map<size_t, size_t> live_;   // offset -> size
+    std::mutex mu_;
+};
+
+// Device allocation owned by its holder (hipMalloc, or a block of an Arena);
+// moves, never copies.
 template <typename T>
 class DevArray {
 public:
     DevArray() = default;
-    explicit DevArray(size_t count) { alloc(count); }
+    explicit DevArray(size_t count, Arena* arena = nullptr) { alloc(count, arena); }
     ~DevArray() { release(); }
     DevArray(const DevArray&) = delete;
     DevArray& operator=(const DevArray&) = delete;
-    DevArray(DevArray&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevArray(DevArray&& o) noexcept : p_(o.p_), n_(o.n_), arena_(o.arena_)
+    {
+        o.p_ = nullptr; o.n_ = 0; o.arena_ = nullptr;
+    }
     DevArray& operator=(DevArray&& o) noexcept
     {
         if (this != &o) {
             release();
-            p_ = o.p_; n_ = o.n_;
-            o.p_ = nullptr; o.n_ = 0;
+            p_ = o.p_; n_ = o.n_; arena_ = o.arena_;
+            o.p_ = nullptr; o.n_ = 0; o.arena_ = nullptr;
         }
         return *this;
     }
-    void alloc(size_t count)
+    void alloc(size_t count, Arena* arena = nullptr)
     {
-        if (count == n_ && p_) return;
+        if (count == n_ && p_ && arena == arena_) return;
         release();
         if (count == 0) return;
-        PARODE_HIP(hipMalloc(reinterpret_cast<void**>(&p_), count * sizeof(T)));
+        if (arena) {
+            p_ = static_cast<T*>(arena->take(count * sizeof(T)));
+            if (!p_) throw std::bad_alloc();
+            arena_ = arena;
+        } else {
+            PARODE_HIP(hipMalloc(reinterpret_cast<void**>(&p_), count * sizeof(T)));
+        }
         n_ = count;
     }
     void release()
     {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            if (arena_)
+                arena_->give(p_);
+            else
+                (void)hipFree(p_);
+        }
         p_ = nullptr;
         n_ = 0;
+        arena_ = nullptr;
     }
     T* get() const { return p_; }
     size_t size() const { return n_; }
@@ -61,6 +175,7 @@ public:
 private:
     T* p_ = nullptr;
     size_t n_ = 0;
+    Arena* arena_ = nullptr;
 };
 
 inline long next_pow2(long v)

@@ -94,6 +94,14 @@ template <typename TU>
 void abm_startup(hipStream_t s, bool mirror, int n, int m, double h, const float* A,
                  const float* B, const float* x0, const TU* Uh, double* X0);
 
+// ------------------------------------------------------------ input samples
+// out[s][j] = a sin(g) + b cos(g) + c + d t (+ sigma * N(0,1) keyed by
+// (seed, s0 + s, j)), t = t0 + (s0 + s) dt, g = w t + phi, coefficient row
+// j = (a, b, c, d, w, phi); FP64, S rows of m.  Device-side inputs of served
+// requests whose u is a sinusoid family.
+void sample_family(hipStream_t s, int m, const double* coef, double t0, double dt, long s0,
+                   long S, double sigma, uint64_t seed, double* out);
+
 // Copies a typed vector into FP64.
 template <typename T>
 void to_f64(hipStream_t s, long count, const T* in, double* out);

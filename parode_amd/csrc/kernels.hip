@@ -1256,5 +1256,49 @@ void to_f64(hipStream_t s, long count, const T* in, double* out)
 
 template void to_f64<float>(hipStream_t, long, const float*, double*);
 
+// ----------------------------------------------------------- input samples
+__device__ inline uint64_t mix64(uint64_t& x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_sample_family(int m, const double* __restrict__ coef, double t0, double dt,
+                                long s0, long S, double sigma, uint64_t seed,
+                                double* __restrict__ out)
+{
+    const long total = S * m;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const long sl = i / m;
+        const int j = (int)(i - sl * m);
+        const double* k = coef + 6 * j;
+        const double t = t0 + (double)(s0 + sl) * dt;
+        const double g = fma(k[4], t, k[5]);
+        double u = fma(k[3], t, fma(k[1], cos(g), fma(k[0], sin(g), k[2])));
+        if (sigma != 0.0) {
+            // Box-Muller on two counter-keyed splitmix64 draws
+            uint64_t st = seed * 0xD1B54A32D192ED03ull + (uint64_t)(s0 + sl) * 0x9E3779B97F4A7C15ull +
+                          (uint64_t)j * 0xA24BAED4963EE407ull;
+            const double u1 = ((double)(mix64(st) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+            const double u2 = (double)(mix64(st) >> 11) * (1.0 / 9007199254740992.0);
+            u += sigma * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        }
+        out[i] = u;
+    }
+}
+
+void sample_family(hipStream_t s, int m, const double* coef, double t0, double dt, long s0,
+                   long S, double sigma, uint64_t seed, double* out)
+{
+    const long total = S * m;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_sample_family, dim3(blocks_for(total, 256, 8192)), dim3(256), 0, s, m,
+                       coef, t0, dt, s0, S, sigma, seed, out);
+    check_launch("sample_family");
+}
+
 }  // namespace k
 }  // namespace parode

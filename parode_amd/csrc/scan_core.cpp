@@ -5,7 +5,8 @@
 
 namespace parode {
 
-ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt)
+ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt,
+                   Arena* arena)
     : method_(method), n_(n), m_(m), p_(p), na_(method == kABM ? 4 * n : n), zC_(zC), zD_(zD),
       nPadMax_(next_pow2(nPadMax < 1 ? 1 : nPadMax)), L_(ilog2(nPadMax_)), opt_(opt)
 {
@@ -14,35 +15,35 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     const size_t na2 = (size_t)na_ * na_;
     const int kq = method == kABM ? 5 : 3;
     if (opt_.precision == kPlain32) {
-        prod32_.alloc(4 * nn + 4 * nm);            // A, A2, A3, A4 | B, AB, A2B, A3B (float)
+        prod32_.alloc(4 * nn + 4 * nm, arena);            // A, A2, A3, A4 | B, AB, A2B, A3B (float)
     } else {
-        prod64_.alloc(4 * nn + 4 * nm);
-        A64_.alloc(nn);
-        B64_.alloc(nm);
+        prod64_.alloc(4 * nn + 4 * nm, arena);
+        A64_.alloc(nn, arena);
+        B64_.alloc(nm, arena);
     }
-    Bbar64_.alloc((size_t)n * kq * (m > 0 ? m : 1));
-    Bbar32_.alloc((size_t)n * kq * (m > 0 ? m : 1));
-    BbT64_.alloc((size_t)n * kq * (m > 0 ? m : 1));
-    BbT32_.alloc((size_t)n * kq * (m > 0 ? m : 1));
-    CT_.alloc((size_t)n * (p > 0 ? p : 1));
-    DT_.alloc((size_t)(m > 0 ? m : 1) * (p > 0 ? p : 1));
-    PT64_.alloc(na2 * (L_ + 1));
+    Bbar64_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
+    Bbar32_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
+    BbT64_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
+    BbT32_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
+    CT_.alloc((size_t)n * (p > 0 ? p : 1), arena);
+    DT_.alloc((size_t)(m > 0 ? m : 1) * (p > 0 ? p : 1), arena);
+    PT64_.alloc(na2 * (L_ + 1), arena);
     int n32 = 0;
     for (int l = 0; l < L_; ++l)
         if (!level_is_f64(l)) n32 = l + 1;
-    if (n32 > 0) PT32_.alloc(na2 * n32);
+    if (n32 > 0) PT32_.alloc(na2 * n32, arena);
     if (n32 > 0 && k::level_mfma_supported(na_)) {
         nap_ = (na_ + 31) / 32 * 32;
         nrp_ = k::mfma_table_rows(na_);
-        PTm_.alloc((size_t)nap_ * nrp_ * n32);
+        PTm_.alloc((size_t)nap_ * nrp_ * n32, arena);
     }
     if (xIsDouble())
-        X64_.alloc((size_t)nPadMax_ * na_);
+        X64_.alloc((size_t)nPadMax_ * na_, arena);
     else
-        X32_.alloc((size_t)nPadMax_ * na_);
-    seed_.alloc(na_);
-    tot_.alloc(na_);
-    seedNext_.alloc(na_);
+        X32_.alloc((size_t)nPadMax_ * na_, arena);
+    seed_.alloc(na_, arena);
+    tot_.alloc(na_, arena);
+    seedNext_.alloc(na_, arena);
 }
 
 size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt)
@@ -60,7 +61,7 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
         b += 4 * (size_t)((na + 31) / 32 * 32) * ((na + 255) / 256 * 256) * L;
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
     b += 24 * na;
-    return b;
+    return b + 20 * Arena::kAlign;   // block rounding when carved from an Arena
 }
 
 size_t ScanCore::footprint() const
@@ -307,6 +308,42 @@ void ScanCore::abm_startup(hipStream_t s, const float* dA, const float* dB, cons
         k::abm_startup<double>(s, mirror, n_, m_, h, dA, dB, dx0, (const double*)dUh, dX0);
     else
         k::abm_startup<float>(s, mirror, n_, m_, h, dA, dB, dx0, (const float*)dUh, dX0);
+}
+
+// ---------------------------------------------------------------- one solve
+void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, const float* dB,
+                     const float* dC, const float* dD, const float* dx0, const void* dU,
+                     bool uDouble, float* dY, hipEvent_t* marks)
+{
+    const long nScan = method_ == kRK4 ? points : points - 3;
+    const long nPad = next_pow2(nScan);
+    if (nScan < 1 || nPad > nPadMax_) throw std::invalid_argument("ScanCore::solve: bad length");
+    auto tick = [&](int i) {
+        if (marks) PARODE_HIP(hipEventRecord(marks[i], s));
+    };
+    tick(0);
+    build_map(s, dA, dB, h);
+    build_powers(s);
+    tick(1);
+    float* yScan = dY;
+    if (method_ == kRK4) {
+        k::to_f64<float>(s, n_, dx0, seed());
+        leaves(s, dU, uDouble, 2, 0, 1, nScan, nPad);
+    } else {
+        abm_startup(s, dA, dB, dx0, dU, uDouble, h, seed());
+        startup_outputs(s, dC, dD, seed(), dU, uDouble, dY);
+        yScan = dY + 3L * nOut();
+        leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
+    }
+    tick(2);
+    sweep_up(s, nScan, nPad);
+    tick(3);
+    sweep_top(s, nScan, nPad, seed(), tot());
+    tick(4);
+    sweep_down(s, nScan, nPad);
+    tick(5);
+    outputs(s, dC, dD, dU, uDouble, 2, method_ == kRK4 ? 0 : 6, nScan, yScan);
+    tick(6);
 }
 
 }  // namespace parode

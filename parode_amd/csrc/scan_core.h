@@ -31,7 +31,9 @@ struct ScanOptions {
 
 class ScanCore {
 public:
-    ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt);
+    // Buffers come from hipMalloc, or from `arena` when given (serving).
+    ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt,
+             Arena* arena = nullptr);
 
     int method() const { return method_; }
     int n() const { return n_; }
@@ -66,6 +68,16 @@ public:
                          const void* dU, bool uDouble, float* dY);
     void abm_startup(hipStream_t s, const float* dA, const float* dB, const float* dx0,
                      const void* dUh, bool uDouble, double h, double* dX0);
+
+    // The whole solve of one system with every array already on the device:
+    // step map, powers, (ABM start-up,) leaves, sweeps, top, outputs, for
+    // `points` outputs (scan length points, or points - 3 for ABM, at most
+    // nPadMax).  Samples on the h/2 grid, (2*points+1) rows of m.  marks, when
+    // given, are 7 events recorded before each of the six phases and after the
+    // last.  Asynchronous on s.
+    void solve(hipStream_t s, long points, double h, const float* dA, const float* dB,
+               const float* dC, const float* dD, const float* dx0, const void* dU, bool uDouble,
+               float* dY, hipEvent_t* marks = nullptr);
 
     double* seed() const { return seed_.get(); }
     double* tot() const { return tot_.get(); }

@@ -1,0 +1,88 @@
+// Continuous-batching request loop for LTI solves on one GPU.
+//
+// Requests (one system each: method, shape, horizon, matrices, inputs) are
+// submitted from any thread and run on a worker thread that owns the GPU
+// work.  The worker admits queued requests in arrival order whenever a lane
+// (a HIP stream) has room and the device arena can hold the request's
+// footprint (scan trajectory, FP64 power table, level tables, inputs and
+// outputs).  An admitted request's whole solve is enqueued on its lane without
+// host synchronisation, so requests on different lanes overlap on the device
+// and new arrivals are admitted while earlier ones run.  A finished request's
+// outputs are copied to the caller's buffer and its memory returns to the
+// arena.  Inputs are either host samples on the h/2 grid or the coefficients
+// of the family u_j = a sin(w t + phi) + b cos(w t + phi) + c + d t (+ noise),
+// sampled in FP64 on the device.
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "hip_util.h"
+#include "scan_core.h"
+
+namespace parode {
+
+struct ServeRequest {
+    int solver = 0, n = 0, m = 0, p = 0;
+    bool zC = false, zD = false;
+    long points = 0;
+    double t0 = 0.0, h = 0.0;
+    const float *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr, *x0 = nullptr;
+    const double* coef = nullptr;     // m x 6, used when samples is null
+    double sigma = 0.0;
+    uint64_t seed = 0;
+    const double* samples = nullptr;  // host, (2*points+1) x m
+    float* Y = nullptr;               // host, points x nOut
+};
+
+struct ServeStats {
+    long submitted = 0, completed = 0, failed = 0, running = 0, queued = 0;
+    long peakRunning = 0;
+    size_t arenaBytes = 0, arenaPeak = 0;
+};
+
+class Server {
+public:
+    Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    // Returns a ticket (> 0); copies the matrices and coefficients, so the
+    // caller keeps only `samples` and `Y` alive until the ticket completes.
+    long submit(const ServeRequest& r);
+    // Blocks until the ticket completes; returns its status (0 or an
+    // ErrorCode) and the times from submission to completion and to admission.
+    int wait(long ticket, double* latencyMs, double* queueMs);
+    void drain();
+    ServeStats stats();
+    static size_t footprint(const ServeRequest& r, int precision);
+
+private:
+    struct Job;
+    using Clock = std::chrono::steady_clock;
+    void loop();
+    bool admit(Job& j, int lane);
+    void finish(Job& j, int status);
+
+    int device_ = 0, precision_ = 0, depth_ = 2;
+    std::vector<hipStream_t> lanes_;
+    std::vector<int> load_;
+    std::unique_ptr<Arena> arena_, pinned_;
+    std::mutex mu_;
+    std::condition_variable work_cv_, done_cv_;
+    std::deque<std::shared_ptr<Job>> queue_;
+    std::vector<std::shared_ptr<Job>> running_;
+    std::unordered_map<long, std::shared_ptr<Job>> jobs_;
+    long next_ = 1;
+    bool stop_ = false;
+    ServeStats st_;
+    std::thread worker_;
+};
+
+}  // namespace parode

@@ -101,3 +101,15 @@ def test_no_gpu_paths():
     assert api.get_device_name(0) == ("", -5)
     assert api.initialize_selected_gpus([0]) == -1
     api.close_gpu()
+
+
+def test_server_footprint_and_no_gpu_create():
+    f = api.Server.footprint(0, 8, 2, 2, 4096)
+    assert f > 4096 * 8 * 4
+    assert api.Server.footprint(1, 8, 2, 2, 3) == -1
+    assert api.Server.footprint(1, 8, 2, 2, 4096) > f
+    if not HAS_GPU:
+        with pytest.raises(api.ParODEError):
+            api.Server(lanes=1, device_bytes=1 << 20, pinned_bytes=1 << 20)
+    with pytest.raises(api.ParODEError):
+        api.Server(lanes=0)
