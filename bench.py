@@ -66,6 +66,168 @@ def cpu_baseline_leg(s, U64, y_gpu, target_s=12.0, full_reference=True):
     return out, acc
 
 
+def _torch_samples(torch, s, dev, dtype):
+    """u(t0 + k h/2), k = 0 .. 2N, of the system's sinusoid family, on the device."""
+    k = torch.arange(2 * s.steps + 1, dtype=torch.float64, device=dev)
+    t = (s.t0 + k * (s.h / 2.0))[:, None]
+    c = torch.from_numpy(np.ascontiguousarray(s.ucoef)).to(dev)
+    g = c[:, 4][None, :] * t + c[:, 5][None, :]
+    U = c[:, 0] * torch.sin(g) + c[:, 1] * torch.cos(g) + c[:, 2] + c[:, 3] * t
+    return U.to(dtype).contiguous()
+
+
+def batch_legs(args, torch, api, synth):
+    """Batch of independent systems (512 by default: RK4/ABM alternating, n
+    log-uniform on [4, 256], N log-uniform on [2^10, 2^18]).
+
+    plain  : FP32 plain precision, one system per solve, the solves queued back
+             to back on one stream, inputs and outputs resident in HBM.
+    served : the continuous-batching server, mixed precision, all requests
+             submitted at once; end to end (host matrices in, host outputs
+             out) from the first submission to the last completion.
+    stream : the server under Poisson arrivals from the same distribution;
+             p50/p99 latency from submission to outputs on the host."""
+    dev = torch.device("cuda:0")
+    systems = synth.batch_config(args.batch)
+    total_steps = float(sum(s.steps for s in systems))
+    weighted = float(sum(s.steps * s.n for s in systems))
+    out = {"systems": len(systems), "state_steps": total_steps}
+
+    # ---- plain
+    plans, bufs = [], []
+    for s in systems:
+        pl = api.Plan(s.method, s.n, s.m, s.p, 0, 0, s.steps, s.h, api.PLAIN32, False)
+        tens = [torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+                for v in (s.A, s.B, s.C, s.D, s.x0)]
+        tens.append(_torch_samples(torch, s, dev, torch.float32))
+        tens.append(torch.empty((s.steps, s.p), dtype=torch.float32, device=dev))
+        plans.append(pl)
+        bufs.append(tens)
+    stream = torch.cuda.Stream(device=dev)
+    sp = stream.cuda_stream
+
+    def run_plain():
+        for pl, t in zip(plans, bufs):
+            pl.execute(*(x.data_ptr() for x in t), sp)
+
+    run_plain()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(1, args.batch_reps)
+    ev0.record(stream)
+    for _ in range(reps):
+        run_plain()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = ev0.elapsed_time(ev1) / reps
+    y_plain = [t[-1].cpu().numpy() for t in bufs[:8]]
+    for pl in plans:
+        pl.close()
+    del plans, bufs
+    torch.cuda.empty_cache()
+    out["plain"] = {"ms": plain_ms, "state_steps_per_s": total_steps / (plain_ms * 1e-3),
+                    "n_weighted_per_s": weighted / (plain_ms * 1e-3), "precision": "plain fp32"}
+
+    # ---- served (all at once)
+    need = sum(api.Server.footprint(s.method, s.n, s.m, s.p, s.steps) for s in systems)
+    srv = api.Server(lanes=args.lanes, device_bytes=int(need * 1.1) + (1 << 30),
+                     pinned_bytes=2 << 30, precision=api.MIXED)
+
+    def serve_all():
+        t0 = time.perf_counter()
+        tickets = [srv.submit(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h, coef=s.ucoef)
+                   for s in systems]
+        lat = []
+        for t, _ in tickets:
+            st, l, q = srv.wait(t)
+            assert st == 0, st
+            lat.append(l)
+        return time.perf_counter() - t0, lat, [Y for _, Y in tickets]
+
+    serve_all()  # warm-up (library and arena first touch)
+    wall, lat, ys = min((serve_all() for _ in range(reps)), key=lambda r: r[0])
+    stats = srv.stats()
+    out["served"] = {"ms": wall * 1e3, "state_steps_per_s": total_steps / wall,
+                     "n_weighted_per_s": weighted / wall, "precision": "mixed",
+                     "lanes": args.lanes, "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                                         "p99": float(np.percentile(lat, 99))},
+                     "peak_running": stats["peak_running"],
+                     "arena_peak_bytes": stats["arena_peak"]}
+    out["served_vs_plain"] = out["served"]["state_steps_per_s"] / out["plain"]["state_steps_per_s"]
+
+    # ---- Poisson stream
+    if args.stream_seconds > 0:
+        import threading
+        rng = np.random.default_rng(7)
+        arrivals, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / args.rate)
+            if t > args.stream_seconds:
+                break
+            arrivals.append((t, int(rng.integers(len(systems)))))
+        done, lock = [], threading.Lock()
+        pending = []
+
+        def waiter():
+            while True:
+                with lock:
+                    item = pending.pop(0) if pending else None
+                if item is None:
+                    time.sleep(0.0005)
+                    continue
+                if item == "stop":
+                    return
+                tk, idx = item
+                st, l, q = srv.wait(tk)
+                done.append((st, l, q, systems[idx].steps))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for ta, idx in arrivals:
+            dt = start + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            s = systems[idx]
+            tk, _ = srv.submit(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h, coef=s.ucoef)
+            with lock:
+                pending.append((tk, idx))
+        with lock:
+            pending.append("stop")
+        th.join()
+        wall_s = time.perf_counter() - start
+        lat_s = [d[1] for d in done]
+        out["stream"] = {"rate_per_s": args.rate, "seconds": args.stream_seconds,
+                         "requests": len(done), "failed": sum(1 for d in done if d[0] != 0),
+                         "state_steps_per_s": sum(d[3] for d in done) / wall_s,
+                         "latency_ms": {"p50": float(np.percentile(lat_s, 50)),
+                                        "p99": float(np.percentile(lat_s, 99))},
+                         "queue_ms_p99": float(np.percentile([d[2] for d in done], 99))}
+    srv.close()
+    return out, systems, ys, y_plain
+
+
+def batch_accuracy(systems, ys, y_plain, count):
+    """Max relative error vs the FP64 oracle over a fixed sample of the served
+    results (every system when count >= len(systems)), and the plain results
+    of the first systems vs their served ones."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from parode_amd import synth
+    idx = sorted(range(len(systems)), key=lambda i: systems[i].steps * systems[i].n_scan ** 2)
+    pick = idx[:count] if count < len(systems) else idx
+    worst = 0.0
+    f = lambda a: a.astype(np.float64)
+    for i in pick:
+        s = systems[i]
+        y64 = orc.simulate_f64(s.method, f(s.A), f(s.B), f(s.C), f(s.D), f(s.x0),
+                               synth.samples_f64(s), s.steps, s.h)
+        worst = max(worst, orc.max_rel_err(ys[i], y64))
+    plain_vs_served = max(orc.max_rel_err(yp, f(ys[i])) for i, yp in enumerate(y_plain))
+    return {"served_max_rel_err_vs_fp64": worst, "checked_systems": len(pick),
+            "plain_vs_served_max_rel": plain_vs_served}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +238,12 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--batch", type=int, default=512, help="systems in the batch legs (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=2)
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrivals per second")
+    ap.add_argument("--stream-seconds", type=float, default=10.0)
+    ap.add_argument("--check-systems", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -156,12 +324,19 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plan_bytes": plan.footprint,
     }
+    plan.close()
+    del tA, tB, tC, tD, tx, tU
+    torch.cuda.empty_cache()
+    if args.batch > 0:
+        bl, systems, ys, y_plain = batch_legs(args, torch, api, synth)
+        if not args.no_check:
+            bl.update(batch_accuracy(systems, ys, y_plain, args.check_systems))
+        line["batch"] = bl
     if not args.no_cpu or not args.no_check:
         cb, acc = cpu_baseline_leg(s, U64, y, full_reference=not args.no_check)
         if not args.no_cpu:
             line["cpu_baseline"] = cb
         line.update(acc)
-    plan.close()
     print(json.dumps(line))
 
 

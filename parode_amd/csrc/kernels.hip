@@ -203,6 +203,14 @@ void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out)
 }
 
 // Zero/constant fill as a kernel (graph-capturable; no memset nodes).
+template <typename T>
+__global__ void k_fill(long count, T* __restrict__ out)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = (T)0;
+}
+
 __global__ void k_fill_f64(long count, double v, double* __restrict__ out)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -444,9 +452,11 @@ struct Acc {
 };
 
 // A work-group takes EB consecutive elements per thread group: the inputs of
-// its elements are staged in LDS, and each thread forms one row r of all EB
-// leaves, so every Bbar value it loads is used EB times.  Threads are grouped
-// by 2^tpeLog (>= na, <= 256) per element block; index math is shifts.
+// its elements are staged in LDS, and each thread forms one row r (< n) of all
+// EB leaves, so every Bbar value it loads is used EB times.  Thread groups are
+// 2^tpeLog >= n wide (<= 256); index math is shifts.  Rows of X outside
+// [off, off + n) (the zero blocks of the ABM companion state) are written by
+// all threads of the work-group together.
 constexpr int kLeafEB = 8;
 
 template <typename T, typename TU, typename TB>
@@ -464,35 +474,40 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
     const int groups = 256 >> tpeLog;
     const int grp = threadIdx.x >> tpeLog, lane = threadIdx.x & (tpe - 1);
     const int km = kq * m;
+    const int nz = na - n;                            // zero rows per element
     const long perBlock = (long)groups * kLeafEB;
     for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
+        const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
         // stage the inputs of the block's elements
-        for (int t = threadIdx.x; t < perBlock * km; t += 256) {
-            const int le = t / km, c = t % km, q = c / (m > 0 ? m : 1), j = c % (m > 0 ? m : 1);
+        for (int t = threadIdx.x; t < cnt * km; t += 256) {
+            const int le = t / km, c = t - le * km;
+            const int q = c / m, j = c - q * m;
             const long e = eb + le;
-            Us[t] = (e < nReal && m > 0) ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
+            Us[t] = e < nReal ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
+        }
+        // zero rows [0, off) and [off + n, na) of the block's elements
+        for (int t = threadIdx.x; t < cnt * nz; t += 256) {
+            const int le = t / nz, z = t - le * nz;
+            X[(eb + le) * na + (z < off ? z : z + n)] = (T)0;
         }
         __syncthreads();
         const long e0 = eb + (long)grp * kLeafEB;
         const TA* ug = Us + (long)grp * kLeafEB * km;
-        for (int rho = lane; rho < na; rho += tpe) {
-            const int r = rho - off;
+        for (int r = lane; r < n && e0 < nPad; r += tpe) {
             TA v[kLeafEB];
 #pragma unroll
             for (int u = 0; u < kLeafEB; ++u) v[u] = 0;
-            if (r >= 0 && r < n) {
-                for (int c = 0; c < km; ++c) {
-                    const TA b = (TA)BbT[(long)c * n + r];
+            for (int c = 0; c < km; ++c) {
+                const TA b = (TA)BbT[(long)c * n + r];
 #pragma unroll
-                    for (int u = 0; u < kLeafEB; ++u) {
-                        TA pr = b * ug[u * km + c];
-                        v[u] = v[u] + pr;
-                    }
+                for (int u = 0; u < kLeafEB; ++u) {
+                    TA pr = b * ug[u * km + c];
+                    v[u] = v[u] + pr;
                 }
             }
 #pragma unroll
             for (int u = 0; u < kLeafEB; ++u)
-                if (e0 + u < nPad) X[(e0 + u) * na + rho] = (T)v[u];
+                if (e0 + u < nPad) X[(e0 + u) * na + off + r] = (T)v[u];
         }
         __syncthreads();
     }
@@ -503,11 +518,17 @@ void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, lo
             const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X)
 {
     using TA = decltype(TB(0) * TU(0));
+    if (m <= 0) {   // no inputs: every leaf is zero
+        hipLaunchKernelGGL((k_fill<T>), dim3(blocks_for(nPad * na, 256, 8192)), dim3(256), 0, s,
+                           nPad * na, X);
+        check_launch("leaves(zero)");
+        return;
+    }
     int tpeLog = 0;
-    while ((1 << tpeLog) < na && tpeLog < 8) ++tpeLog;
+    while ((1 << tpeLog) < n && tpeLog < 8) ++tpeLog;
     const long perBlock = (256L >> tpeLog) * kLeafEB;
     const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
-    const size_t lds = (size_t)perBlock * kq * (m > 0 ? m : 1) * sizeof(TA);
+    const size_t lds = (size_t)perBlock * kq * m * sizeof(TA);
     hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
                        lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, Bbar, U, sL, oL, rs, X);
     check_launch("leaves");
