@@ -244,6 +244,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--stream-seconds", type=float, default=10.0)
     ap.add_argument("--check-systems", type=int, default=48)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device to device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -252,6 +254,16 @@ def main():
 
     import torch
     from parode_amd import api, synth
+
+    if args.copy_probe:
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4}))
+        return
 
     prec = {"plain": api.PLAIN32, "mixed": api.MIXED, "fp64": api.FP64}[args.precision]
     s = synth.config(args.config, steps=args.points or None)

@@ -137,7 +137,7 @@ void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double
               bool mfma)
 {
     if (mfma) {
-        dim3 grid((c + 31) / 32, (r + 31) / 32);
+        dim3 grid((c + 15) / 16, (r + 15) / 16);
         hipLaunchKernelGGL(k_gemm_f64_mfma, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
     } else {
         dim3 grid((c + 63) / 64, (r + 63) / 64);
@@ -146,9 +146,13 @@ void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double
     check_launch("gemm_f64");
 }
 
-// FP64 GEMM on the matrix cores (v_mfma_f64_16x16x4_f64): a work-group of four
-// waves owns a 32x32 tile of Z, one 16x16 quarter per wave; 32-deep slices of
-// X and Y are staged through LDS.
+// FP64 GEMM on the matrix cores (v_mfma_f64_16x16x4_f64), Z = X Y, for the
+// small products of the step map and the squarings of the power table.  A
+// work-group of four waves owns one 16x16 tile of Z and the waves split the K
+// range in four: each wave issues the loads of 16 k-steps before their MFMAs,
+// so a 256-deep product waits on one round of loads, not one per k-slice.  The
+// four partial tiles are summed through LDS.  Operands come from L2 (the
+// matrices are at most a few MiB).
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
@@ -156,35 +160,41 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
                                                        const double* __restrict__ Y,
                                                        double* __restrict__ Z)
 {
-    constexpr int T = 32, KB = 32;
-    __shared__ double As[T][KB + 1];
-    __shared__ double Bs[KB][T + 1];
+    __shared__ double part[3][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
     const int l16 = lane & 15, q4 = lane >> 4;
+    const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+    const int ksteps = (kk + 3) / 4;
+    const int per = (ksteps + 3) / 4;
+    const int kb = wave * per, ke = (kb + per) < ksteps ? kb + per : ksteps;
+    const int ia = i0 + l16, jb = j0 + l16;
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < kk; k0 += KB) {
-        for (int t = threadIdx.x; t < T * KB; t += 256) {
-            const int a = t / KB, b = t % KB;
-            As[a][b] = (i0 + a < r && k0 + b < kk) ? X[(long)(i0 + a) * kk + k0 + b] : 0.0;
-            const int bk = t / T, bj = t % T;
-            Bs[bk][bj] = (k0 + bk < kk && j0 + bj < c) ? Y[(long)(k0 + bk) * c + j0 + bj] : 0.0;
-        }
-        __syncthreads();
+    constexpr int U = 16;
+    for (int k = kb; k < ke; k += U) {
+        double a[U], b[U];
 #pragma unroll
-        for (int k4 = 0; k4 < KB; k4 += 4) {
-            const double a = As[wr * 16 + l16][k4 + q4];
-            const double b = Bs[k4 + q4][wc * 16 + l16];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        for (int u = 0; u < U; ++u) {
+            const int kx = (k + u) * 4 + q4;
+            const bool ok = k + u < ke && kx < kk;
+            a[u] = (ok && ia < r) ? X[(long)ia * kk + kx] : 0.0;
+            b[u] = (ok && jb < c) ? Y[(long)kx * c + jb] : 0.0;
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
-    // v_mfma_f64_16x16x4_f64 returns D[i][j] in lane j + 16 (i % 4), register i / 4
+    if (wave > 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * 16 + 4 * v + q4, j = j0 + wc * 16 + l16;
-        if (i < r && j < c) Z[(long)i * c + j] = acc[v];
+        for (int v = 0; v < 4; ++v) part[wave - 1][v][lane] = acc[v];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // v_mfma_f64_16x16x4_f64 returns D[i][j] in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double sum = acc[v] + part[0][v][lane] + part[1][v][lane] + part[2][v][lane];
+            const int i = i0 + 4 * v + q4, j = j0 + l16;
+            if (i < r && j < c) Z[(long)i * c + j] = sum;
+        }
     }
 }
 
@@ -919,67 +929,57 @@ __device__ __forceinline__ void top_update(bool down, int na, long d, long c0, i
     }
 }
 
-// Update for na >= 64: lanes r16 + 16 kq of a wave share row i = 16 w + r16
-// and split the j range in four (kq), so each thread's chain of table loads
-// is a quarter as long; the four partial sums meet by cross-lane shuffles.
-constexpr int kTopKsCols = 16;
+// Update for na >= 16 on the FP64 matrix cores: Out[i][c] = sum_j P[i][j] S[c][j]
+// for the cc <= 16 staged columns, A = P from the transposed FP64 table (lane
+// l: i = 16 ib + l % 16, j = 4 k + l / 16) and B = S from LDS (pitch SP, odd).
+// Wave w owns the 16-row blocks ib = w, w + 16, ...; each wave issues the loads
+// of 8 k-steps before their MFMAs.
+constexpr int kTopMfmaCols = 16;
 
-template <typename TX, typename TA>
-__device__ __forceinline__ void top_update_ksplit(bool down, int na, long d, long c0, int cc,
-                                                  const TA* __restrict__ P, const double* Bsh,
-                                                  TX* X)
+template <typename TX>
+__device__ __forceinline__ void top_update_mfma(bool down, int na, int SP, long d, long c0, int cc,
+                                                const double* __restrict__ PT, const double* Ssh,
+                                                TX* X)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r16 = lane & 15, kq = lane >> 4;
-    const int jq = (na + 3) / 4;
-    const int jlo = kq * jq, jhi = (jlo + jq) < na ? jlo + jq : na;
-    for (int rp = 0; rp < na; rp += 256) {
-        const int i = rp + w * 16 + r16;
-        const bool live = i < na;
-        const int ic = live ? i : 0;
-        TA acc[kTopKsCols];
+    const int l16 = lane & 15, q4 = lane >> 4;
+    const int nb = (na + 15) / 16, ks = (na + 3) / 4;
+    for (int ib = w; ib < nb; ib += kTopThreads / 64) {
+        const int ia = ib * 16 + l16;
+        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+        constexpr int U = 8;
+        for (int k = 0; k < ks; k += U) {
+            double a[U], b[U];
 #pragma unroll
-        for (int c = 0; c < kTopKsCols; ++c) acc[c] = 0;
-        int j = jlo;
-        for (; j + 4 <= jhi; j += 4) {
-            TA pj[4];
+            for (int u = 0; u < U; ++u) {
+                const int j = (k + u) * 4 + q4;
+                a[u] = (j < na && ia < na) ? PT[(long)j * na + ia] : 0.0;
+                b[u] = (j < na && l16 < cc) ? Ssh[l16 * SP + j] : 0.0;
+            }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = P[(long)(j + u) * na + ic];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int c = 0; c < kTopKsCols; ++c)
-                    if (c < cc) acc[c] = fma(pj[u], (TA)Bsh[c * na + j + u], acc[c]);
+            for (int u = 0; u < U; ++u)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
-        for (; j < jhi; ++j) {
-            const TA pj = P[(long)j * na + ic];
+        // D[i][c] is in lane c + 16 (i % 4), register i / 4
+        const int c = l16;
+        if (c >= cc) continue;
+        const long R = (c0 + c + 1) * 2 * d - 1, Lp = R - d;
 #pragma unroll
-            for (int c = 0; c < kTopKsCols; ++c)
-                if (c < cc) acc[c] = fma(pj, (TA)Bsh[c * na + j], acc[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < kTopKsCols; ++c) {
-            acc[c] += __shfl_xor(acc[c], 16);
-            acc[c] += __shfl_xor(acc[c], 32);
-        }
-        if (live && kq == 0) {
-#pragma unroll
-            for (int c = 0; c < kTopKsCols; ++c) {
-                if (c >= cc) continue;
-                const long R = (c0 + c + 1) * 2 * d - 1, Lp = R - d;
-                if (!down) {
-                    X[R * na + i] = (TX)((TA)X[R * na + i] + acc[c]);
-                } else {
-                    const TA xl = (TA)X[Lp * na + i];
-                    X[Lp * na + i] = (TX)Bsh[c * na + i];
-                    X[R * na + i] = (TX)(acc[c] + xl);
-                }
+        for (int v = 0; v < 4; ++v) {
+            const int i = ib * 16 + 4 * v + q4;
+            if (i >= na) continue;
+            if (!down) {
+                X[R * na + i] = (TX)((double)X[R * na + i] + acc[v]);
+            } else {
+                const double xl = (double)X[Lp * na + i];
+                X[Lp * na + i] = (TX)Ssh[c * SP + i];
+                X[R * na + i] = (TX)(acc[v] + xl);
             }
         }
     }
 }
 
-template <typename TX>
+template <typename TX, bool MFMA>
 __global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int l64, long nReal,
                                                      const float* __restrict__ PT32,
                                                      const double* __restrict__ PT64,
@@ -987,8 +987,9 @@ __global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int 
                                                      const double* __restrict__ seed,
                                                      double* __restrict__ tot, int CC)
 {
-    extern __shared__ double Bsh[];   // CC x na staged source columns
+    extern __shared__ double Bsh[];   // CC x SP staged source columns
     const int t = threadIdx.x;
+    const int SP = MFMA ? na + 1 : na;
     const int rowThreads = na < kTopThreads ? na : kTopThreads;
     const int groups = kTopThreads / rowThreads;
     const int i0 = t % rowThreads, g = t / rowThreads;
@@ -1004,16 +1005,11 @@ __global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int 
                 for (int e = t; e < cc * na; e += kTopThreads) {
                     const int c = e / na, j = e % na;
                     const long R = (c0 + c + 1) * 2 * d - 1;
-                    Bsh[c * na + j] = (double)X[(down ? R : R - d) * na + j];
+                    Bsh[c * SP + j] = (double)X[(down ? R : R - d) * na + j];
                 }
                 __syncthreads();
-                if (na >= 64) {
-                    if (l >= l64)
-                        top_update_ksplit<TX, double>(down, na, d, c0, cc, PT64 + (long)l * na2,
-                                                      Bsh, X);
-                    else
-                        top_update_ksplit<TX, float>(down, na, d, c0, cc, PT32 + (long)l * na2,
-                                                     Bsh, X);
+                if (MFMA) {
+                    top_update_mfma<TX>(down, na, SP, d, c0, cc, PT64 + (long)l * na2, Bsh, X);
                 } else if (g < groups) {
                     if (l >= l64)
                         top_update<TX, double>(down, na, d, c0, cc, i0, rowThreads, g, groups,
@@ -1040,15 +1036,20 @@ template <typename TX>
 void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float* PT32,
          const double* PT64, TX* X, const double* seed, double* tot)
 {
-    // columns per pass: what the threads can hold (groups x kTopMaxCols) and
-    // what 48 KiB of LDS can stage
+    // columns per pass: one MFMA B tile (16) for na >= 16, else what the
+    // threads can hold (groups x kTopMaxCols); at most 48 KiB of LDS staged
     const int rowThreads = na < kTopThreads ? na : kTopThreads;
     const int groups = kTopThreads / rowThreads;
-    int CC = na >= 64 ? std::min(kTopKsCols, 6144 / na) : std::min(groups * kTopMaxCols, 6144 / na);
+    const int SP = na >= 16 ? na + 1 : na;
+    int CC = na >= 16 ? std::min(kTopMfmaCols, 6144 / SP) : std::min(groups * kTopMaxCols, 6144 / na);
     if (CC < 1) CC = 1;
-    size_t lds = (size_t)CC * na * sizeof(double);
-    hipLaunchKernelGGL((k_top<TX>), dim3(1), dim3(kTopThreads), lds, s, na, l0, L, l64, nReal,
-                       PT32, PT64, X, seed, tot, CC);
+    size_t lds = (size_t)CC * SP * sizeof(double);
+    if (na >= 16)
+        hipLaunchKernelGGL((k_top<TX, true>), dim3(1), dim3(kTopThreads), lds, s, na, l0, L, l64,
+                           nReal, PT32, PT64, X, seed, tot, CC);
+    else
+        hipLaunchKernelGGL((k_top<TX, false>), dim3(1), dim3(kTopThreads), lds, s, na, l0, L, l64,
+                           nReal, PT32, PT64, X, seed, tot, CC);
     check_launch("top");
 }
 
