@@ -72,6 +72,16 @@ int ParODE_PlanExecute(void* plan, const float* dA, const float* dB, const float
 int ParODE_PlanExecuteTimed(void* plan, const float* dA, const float* dB, const float* dC,
 							const float* dD, const float* dx0, const void* dSamples,
 							float* dY, void* stream, float* phaseMs);
+/* Records the whole solve with these device pointers into a HIP graph (on
+ * `stream`, which must not be the null stream); ParODE_PlanReplay then
+ * launches it as one graph.  For callers that repeat one solve (same shape,
+ * same buffers, new contents) and want the launch cost of many small kernels
+ * off the critical path.  A new capture replaces the previous one.  The graph
+ * holds no memset nodes. */
+int ParODE_PlanCapture(void* plan, const float* dA, const float* dB, const float* dC,
+					   const float* dD, const float* dx0, const void* dSamples,
+					   float* dY, void* stream);
+int ParODE_PlanReplay(void* plan, void* stream);
 long ParODE_PlanFootprint(void* plan);
 int ParODE_PlanLevels(void* plan);
 int ParODE_PlanDestroy(void* plan);

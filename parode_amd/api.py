@@ -21,6 +21,7 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_PlanCreate", "ParODE_PlanExecute", "ParODE_PlanExecuteTimed",
                "ParODE_PlanFootprint",
                "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes",
+               "ParODE_PlanCapture", "ParODE_PlanReplay",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
                "ParODE_ServerDestroy"]
@@ -85,6 +86,10 @@ def lib():
         L.ParODE_PlanExecute.restype = ctypes.c_int
         L.ParODE_PlanExecuteTimed.argtypes = [_vp] * 9 + [_fp]
         L.ParODE_PlanExecuteTimed.restype = ctypes.c_int
+        L.ParODE_PlanCapture.argtypes = [_vp] * 9
+        L.ParODE_PlanCapture.restype = ctypes.c_int
+        L.ParODE_PlanReplay.argtypes = [_vp, _vp]
+        L.ParODE_PlanReplay.restype = ctypes.c_int
         L.ParODE_PlanFootprint.argtypes = [_vp]
         L.ParODE_PlanFootprint.restype = ctypes.c_long
         L.ParODE_PlanLevels.argtypes = [_vp]
@@ -296,6 +301,17 @@ class Plan:
         r = lib().ParODE_PlanExecute(self.handle, dA, dB, dC, dD, dx0, dU, dY, stream)
         if r != 0:
             raise ParODEError(r, "ParODE_PlanExecute")
+
+    def capture(self, dA, dB, dC, dD, dx0, dU, dY, stream):
+        """Record the solve with these pointers into a HIP graph (stream != 0)."""
+        r = lib().ParODE_PlanCapture(self.handle, dA, dB, dC, dD, dx0, dU, dY, stream)
+        if r != 0:
+            raise ParODEError(r, "ParODE_PlanCapture")
+
+    def replay(self, stream=0):
+        r = lib().ParODE_PlanReplay(self.handle, stream)
+        if r != 0:
+            raise ParODEError(r, "ParODE_PlanReplay")
 
     PHASES = ("map_powers", "leaves", "up_sweep", "top", "down_sweep", "outputs")
 

@@ -22,8 +22,18 @@ struct Plan {
     int device;
     std::unique_ptr<parode::ScanCore> core;
     hipEvent_t ev[8] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
     ~Plan()
     {
+        drop_graph();
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -132,6 +142,52 @@ int ParODE_PlanExecuteTimed(void* plan, const float* dA, const float* dB, const 
         PARODE_HIP(hipEventSynchronize(pl->ev[kPhases]));
         for (int i = 0; i < kPhases; ++i)
             PARODE_HIP(hipEventElapsedTime(&phaseMs[i], pl->ev[i], pl->ev[i + 1]));
+        return ParODE_OK;
+    } catch (std::exception& e) {
+        parode::last_error() = e.what();
+        return ParODE_NotEnoughResources;
+    }
+}
+
+int ParODE_PlanCapture(void* plan, const float* dA, const float* dB, const float* dC,
+                       const float* dD, const float* dx0, const void* dSamples, float* dY,
+                       void* stream)
+{
+    Plan* pl = static_cast<Plan*>(plan);
+    if (!pl || !stream) return ParODE_InvalidSystem;
+    if (!dA || !dx0 || !dY || (pl->m > 0 && (!dB || !dSamples)) || (!pl->zC && !dC) ||
+        (!pl->zD && !dD))
+        return ParODE_InvalidSystem;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    try {
+        PARODE_HIP(hipSetDevice(pl->device));
+        pl->drop_graph();
+        PARODE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            run_plan(pl, dA, dB, dC, dD, dx0, dSamples, dY, s, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        PARODE_HIP(hipStreamEndCapture(s, &pl->graph));
+        PARODE_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+        return ParODE_OK;
+    } catch (std::exception& e) {
+        pl->drop_graph();
+        parode::last_error() = e.what();
+        return ParODE_NotEnoughResources;
+    }
+}
+
+int ParODE_PlanReplay(void* plan, void* stream)
+{
+    Plan* pl = static_cast<Plan*>(plan);
+    if (!pl || !pl->exec) return ParODE_InvalidSystem;
+    try {
+        PARODE_HIP(hipSetDevice(pl->device));
+        PARODE_HIP(hipGraphLaunch(pl->exec, static_cast<hipStream_t>(stream)));
         return ParODE_OK;
     } catch (std::exception& e) {
         parode::last_error() = e.what();

@@ -76,3 +76,31 @@ def test_plan_repeated_execution_is_deterministic():
     y1 = _run_plan(s, U, api.PLAIN32)
     y2 = _run_plan(s, U, api.PLAIN32)
     np.testing.assert_array_equal(y1, y2)
+
+
+def test_plan_graph_replay_equals_eager():
+    """A captured solve replayed several times, with eager work between the
+    replays, gives the eager result every time."""
+    s = synth.make_system(1, 12, 2, 3, -5.0, -0.1, 1e-3, 5000, seed=25, noise_sigma=0.1)
+    U = synth.samples_f64(s)
+    plan = api.Plan(s.method, s.n, s.m, s.p, 0, 0, s.steps, s.h, api.MIXED, True)
+    tA, tB, tC, tD, tx = (_dev(v) for v in (s.A, s.B, s.C, s.D, s.x0))
+    tU = _dev(U)
+    tY = torch.zeros((s.steps, s.p), dtype=torch.float32, device="cuda:0")
+    ptrs = (tA.data_ptr(), tB.data_ptr(), tC.data_ptr(), tD.data_ptr(), tx.data_ptr(),
+            tU.data_ptr(), tY.data_ptr())
+    st = torch.cuda.Stream()
+    plan.execute(*ptrs, st.cuda_stream)
+    torch.cuda.synchronize()
+    y_eager = tY.clone()
+    plan.capture(*ptrs, st.cuda_stream)
+    torch.cuda.synchronize()
+    for r in range(3):
+        tY.zero_()
+        torch.cuda.synchronize()
+        plan.replay(st.cuda_stream)
+        torch.cuda.synchronize()
+        assert torch.equal(tY, y_eager), f"replay {r}"
+        plan.execute(*ptrs, st.cuda_stream)      # eager work between replays
+        torch.cuda.synchronize()
+    plan.close()
