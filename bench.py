@@ -24,6 +24,12 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 
+# HBM bytes of one level-0 up-sweep launch of the level kernel at the default
+# workload (2^19 pairs of 256-float columns), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counters, FETCH scaled by 2.0 (what the counter reads for a plain
+# copy of known size): (529458 * 2.0 + 524287) KiB.
+TRAFFIC_LEVEL0_UP = (529458 * 2.0 + 524287) * 1024
+
 
 def level_flops(na, n_scan):
     """Algorithmic flops of the level sweeps: 2 na^2 per pair, pairs of
@@ -329,9 +335,12 @@ def main():
                                f"p={s.p} h={s.h} N={s.steps} ({s.tag})",
                    "precision": args.precision, "points": s.steps, "scan_size": na},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": "level sweeps (k_level)",
-                     "flops_per_solve": flops, "pairs_per_sweep": pairs, "sweep_ms": sweep_ms},
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": TRAFFIC_LEVEL0_UP if (args.config == 1 and not args.points)
+                     else None,
+                     "kernel": "k_level_mfma (every level launch of both sweeps)",
+                     "flops_per_solve": flops, "pairs_per_sweep": pairs, "sweep_ms": sweep_ms,
+                     "algorithmic_bytes_level0_up": 3 * (n_scan // 2) * na * 4},
         "phases_ms": ph,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plan_bytes": plan.footprint,

@@ -45,9 +45,11 @@ void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, do
 // X[e][off + r] = sum_{q<k} Bbar[r][q*m + j] * U[(e*sL + oL + q*rs)*m + j], e < nReal;
 // rows off..off+n of X for e in [nReal, nPad) are zero; other rows of X are
 // zeroed when zeroOther (the ABM zero fill).  T: X type, TU: U type.
+// writeZeros = false leaves the rows outside [off, off + n) unwritten (for a
+// first level that never reads them).
 template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, long nPad,
-            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X);
+            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X, bool writeZeros = true);
 
 // ------------------------------------------------------------ scan levels
 // Up-sweep level with stride d over np pairs: X[R] += P X[L] (R = (k+1)2d-1,
@@ -62,8 +64,14 @@ bool level_mfma_supported(int na);
 int mfma_table_rows(int na);
 void make_level_table_padded(hipStream_t s, int na, int nap, int nrp, const double* P, bool delta,
                              float* PTp);
+// rbeg: first row that is not zero in the level-0 leaves (ABM: 3n; the rows
+// below are not read); jbeg: first source state the up-sweep product needs (a
+// multiple of 32, <= rbeg; 0 for the down-sweep).  Both 0 in general.
+// For na > 256 (level_mfma_splits) the rows are split over work-groups, and
+// the down-sweep first copies the pairs' X[R] into S (np * na floats).
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X);
+                float* X, int jbeg = 0, int rbeg = 0, float* S = nullptr);
+inline bool level_mfma_splits(int na) { return na > 256; }
 
 // Levels [l0, L) of both sweeps and the root in one work-group (levels >= l64
 // from the FP64 table, the others from the FP32 one).

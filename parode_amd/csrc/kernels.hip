@@ -471,7 +471,7 @@ constexpr int kLeafEB = 8;
 
 template <typename T, typename TU, typename TB>
 __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
-                                                long nPad, int tpeLog,
+                                                long nPad, int tpeLog, bool writeZeros,
                                                 const TB* __restrict__ BbT,
                                                 const TU* __restrict__ U, long sL, long oL,
                                                 long rs, T* __restrict__ X)
@@ -495,8 +495,11 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
             const long e = eb + le;
             Us[t] = e < nReal ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
         }
-        // zero rows [0, off) and [off + n, na) of the block's elements
-        for (int t = threadIdx.x; t < cnt * nz; t += 256) {
+        // zero rows [0, off) and [off + n, na) of the block's elements (of its
+        // padding elements only, when the first level does not read them)
+        const long firstZ = writeZeros ? eb : (nReal > eb ? nReal : eb);
+        const int z0 = firstZ - eb < cnt ? (int)(firstZ - eb) * nz : cnt * nz;
+        for (int t = z0 + threadIdx.x; t < cnt * nz; t += 256) {
             const int le = t / nz, z = t - le * nz;
             X[(eb + le) * na + (z < off ? z : z + n)] = (T)0;
         }
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
 
 template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, long nPad,
-            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X)
+            const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X, bool writeZeros)
 {
     using TA = decltype(TB(0) * TU(0));
     if (m <= 0) {   // no inputs: every leaf is zero
@@ -540,21 +543,22 @@ void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, lo
     const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
     const size_t lds = (size_t)perBlock * kq * m * sizeof(TA);
     hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
-                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, Bbar, U, sL, oL, rs, X);
+                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, writeZeros, Bbar, U, sL,
+                       oL, rs, X);
     check_launch("leaves");
 }
 
 template void leaves<float, float, float>(hipStream_t, int, int, int, int, int, long, long,
-                                          const float*, const float*, long, long, long, float*);
+                                          const float*, const float*, long, long, long, float*, bool);
 template void leaves<float, double, double>(hipStream_t, int, int, int, int, int, long, long,
                                             const double*, const double*, long, long, long,
-                                            float*);
+                                            float*, bool);
 template void leaves<double, double, double>(hipStream_t, int, int, int, int, int, long, long,
                                              const double*, const double*, long, long, long,
-                                             double*);
+                                             double*, bool);
 template void leaves<double, float, double>(hipStream_t, int, int, int, int, int, long, long,
                                             const double*, const float*, long, long, long,
-                                            double*);
+                                            double*, bool);
 
 // ------------------------------------------------------------ scan levels
 // One work-group owns TC whole column pairs (all na rows), so the down-sweep
@@ -696,20 +700,37 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // PT is the zero-padded transposed table, [nrp][nap] with nap = 32*ceil(na/32)
 // and nrp a multiple of KC, so fragment loads need no bounds checks; the next
 // k-step's B fragments are loaded while the current MFMAs run.
+// Level 0 of the ABM scan reads leaves that hold only the last n of the 4n
+// companion rows (the leaves leave the zero rows unwritten): rbeg = 3n is the
+// first row of a leaf that is not zero (the X[L] source and the X[R]
+// accumulator of the up-sweep, the X[L] addend of the down-sweep), and jbeg =
+// 32 floor(3n / 32) the first source state the up-sweep product needs.
+// Elsewhere both are 0.
+//
+// For n_a above one work-group's rows (IB = WGI*NIT*32), blockIdx.y picks a
+// block of IB rows i.  The up-sweep stays race-free (it reads X[L], writes
+// X[R]).  The down-sweep reads its source columns from S, a compact copy of
+// the old X[R] ([np][na], made by k_gather_r), because other row blocks write
+// X[R] while this one still reads it.  S is null when one block covers n_a.
 template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC>
 __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, long d, long np,
+                                                    int jbeg, int rbeg,
                                                     const float* __restrict__ PT,
+                                                    const float* __restrict__ S,
                                                     float* __restrict__ X)
 {
     constexpr int WGC = 4 / WGI;
     constexpr int TC = WGC * NCT * 32;
     constexpr int PITCH = KC + 1;
     constexpr int PER = TC * KC / 256;
+    constexpr int IB = WGI * NIT * 32;
     __shared__ float As[TC * PITCH];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wi = wave % WGI, wc = wave / WGI;
     const int l32 = lane & 31, hi = lane >> 5;
     const long k0 = (long)blockIdx.x * TC;
+    const int ib0 = blockIdx.y * IB;
+    const bool fromS = DOWN && S != nullptr;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
     floatx16 acc[NCT][NIT];
 #pragma unroll
@@ -720,8 +741,11 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
     const float* pcol[NIT];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) pcol[it] = PT + (wi + it * WGI) * 32 + l32;
-    for (int j0 = 0; j0 < na; j0 += KC) {
+    for (int it = 0; it < NIT; ++it) {
+        const int col = ib0 + (wi + it * WGI) * 32 + l32;
+        pcol[it] = PT + (col < nap ? col : nap - 1);
+    }
+    for (int j0 = jbeg; j0 < na; j0 += KC) {
         const int kc = (na - j0) < KC ? (na - j0) : KC;
         float v[PER];
 #pragma unroll
@@ -729,7 +753,9 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
             const int t = threadIdx.x + u * 256;
             const int c = t / KC, j = t % KC;
             const long R = (k0 + c + 1) * 2 * d - 1;
-            v[u] = (c < ncol && j < kc) ? X[(DOWN ? R : R - d) * na + j0 + j] : 0.0f;
+            v[u] = (c < ncol && j < kc && (DOWN || j0 + j >= rbeg))
+                       ? (fromS ? S[(k0 + c) * na + j0 + j] : X[(DOWN ? R : R - d) * na + j0 + j])
+                       : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -784,7 +810,7 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
-            const int i = (wi + it * WGI) * 32 + l32;
+            const int i = ib0 + (wi + it * WGI) * 32 + l32;
             if (i >= na) continue;
             float xr[16], xl[16];
 #pragma unroll
@@ -792,8 +818,11 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
                 const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
                 const long R = (k0 + cr + 1) * 2 * d - 1;
                 const bool ok = full || cr < ncol;
-                xr[v] = ok ? X[R * na + i] : 0.0f;
-                xl[v] = (ok && (DOWN || DELTA)) ? X[(R - d) * na + i] : 0.0f;
+                xr[v] = (ok && (DOWN || i >= rbeg))
+                            ? (fromS ? S[(k0 + cr) * na + i] : X[R * na + i])
+                            : 0.0f;
+                xl[v] = (ok && (DOWN || DELTA) && (!DOWN || i >= rbeg)) ? X[(R - d) * na + i]
+                                                                       : 0.0f;
             }
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
@@ -815,28 +844,50 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
 
 int mfma_table_rows(int na);
 
+__global__ void k_gather_r(int na, long d, long np, const float* __restrict__ X,
+                           float* __restrict__ S)
+{
+    const long total = np * na;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long c = e / na;
+        const int j = (int)(e - c * na);
+        S[e] = X[((c + 1) * 2 * d - 1) * na + j];
+    }
+}
+
 template <int NIT, int NCT, int WGI>
 static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
-                        const float* PT, float* X)
+                        int jbeg, int rbeg, const float* PT, float* X, float* S)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
     constexpr int KC = 8192 / TC;
+    constexpr int IB = WGI * NIT * 32;
     const int nap = (na + 31) / 32 * 32, nrp = mfma_table_rows(na);
-    dim3 g((unsigned)((np + TC - 1) / TC)), b(256);
+    const unsigned gy = (unsigned)((na + IB - 1) / IB);
+    if (gy > 1 && down) {
+        if (!S) throw std::invalid_argument("level_mfma: split rows need the gather buffer");
+        hipLaunchKernelGGL(k_gather_r, dim3(blocks_for(np * na, 256, 8192)), dim3(256), 0, s, na,
+                           d, np, X, S);
+        check_launch("gather_r");
+    } else {
+        S = nullptr;
+    }
+    dim3 g((unsigned)((np + TC - 1) / TC), gy), b(256);
     if (down) {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap, nrp, d,
-                               np, PT, X);
+                               np, jbeg, rbeg, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, PT, X);
+                               nrp, d, np, jbeg, rbeg, PT, S, X);
     } else {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, PT, X);
+                               nrp, d, np, jbeg, rbeg, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, PT, X);
+                               nrp, d, np, jbeg, rbeg, PT, S, X);
     }
     check_launch("level_mfma");
 }
@@ -849,27 +900,29 @@ bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
 // Tile choice by scan size; the narrow variant (32 pairs per work-group)
 // takes over when the wide one would leave the 256 CUs short of work.
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X)
+                float* X, int jbeg, int rbeg, float* S)
 {
     if (np <= 0) return;
+    if (jbeg % 32 != 0 || jbeg < 0 || jbeg >= na || rbeg < 0 || rbeg >= na || jbeg > rbeg ||
+        (down && jbeg != 0) || (delta && rbeg != 0))
+        throw std::invalid_argument("level_mfma: bad structured range");
     const int nt = (na + 31) / 32;
     const bool narrow = np < 2L * 256 * 64;
     if (nt <= 1) {
-        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, PT, X);
-        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, PT, X);
+        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
     } else if (nt <= 2) {
-        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, PT, X);
-        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, PT, X);
+        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
     } else if (nt <= 4) {
-        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, PT, X);
-        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, PT, X);
+        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
     } else if (nt <= 8) {
-        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, PT, X);
-        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, PT, X);
-    } else if (nt <= 16) {
-        mfma_launch<4, 1, 4>(s, down, delta, na, d, np, PT, X);
+        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
     } else {
-        mfma_launch<8, 1, 4>(s, down, delta, na, d, np, PT, X);
+        // above 256 states: 256-row blocks side by side (blockIdx.y)
+        mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, S);
     }
 }
 

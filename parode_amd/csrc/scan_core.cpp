@@ -36,6 +36,8 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
         nap_ = (na_ + 31) / 32 * 32;
         nrp_ = k::mfma_table_rows(na_);
         PTm_.alloc((size_t)nap_ * nrp_ * n32, arena);
+        // down-sweep copy of X[R] for the row-split MFMA levels
+        if (k::level_mfma_splits(na_)) Sg_.alloc((size_t)(nPadMax_ / 2 > 0 ? nPadMax_ / 2 : 1) * na_, arena);
     }
     if (xIsDouble())
         X64_.alloc((size_t)nPadMax_ * na_, arena);
@@ -57,8 +59,10 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     b += 24 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
     b += 8 * na * na * (L + 1);
     if (opt.precision != kFp64) b += 4 * na * na * L;
-    if (opt.precision != kFp64 && na >= 17 && na <= 1024)
+    if (opt.precision != kFp64 && na >= 17 && na <= 1024) {
         b += 4 * (size_t)((na + 31) / 32 * 32) * ((na + 255) / 256 * 256) * L;
+        if (k::level_mfma_splits((int)na)) b += 4 * (size_t)(nP / 2 > 0 ? nP / 2 : 1) * na;
+    }
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
     b += 24 * na;
     return b + 20 * Arena::kAlign;   // block rounding when carved from an Arena
@@ -68,7 +72,8 @@ size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
            Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + CT_.bytes() + DT_.bytes() +
-           PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
+           PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + Sg_.bytes() + X32_.bytes() + X64_.bytes() +
+           seed_.bytes() +
            tot_.bytes() + seedNext_.bytes();
 }
 
@@ -168,6 +173,15 @@ void ScanCore::build_powers(hipStream_t s)
 }
 
 // -------------------------------------------------------------------- leaves
+// ABM leaves hold only the last n of the 4n companion rows.  When level 0 of
+// the up-sweep runs on the matrix cores as its own launch, it reads just those
+// rows (and only the matching columns of P), so the leaves skip the zero rows.
+bool ScanCore::structured_level0(long nReal, long nPad) const
+{
+    return method_ == kABM && !xIsDouble() && L_ > 0 && !level_is_f64(0) && !level_is_delta(0) &&
+           (level_kernel() & 1) == 0 && PTm_.get() != nullptr && top_level(nReal, nPad) > 0;
+}
+
 void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs,
                       long nReal, long nPad)
 {
@@ -183,10 +197,12 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
                                              (const float*)dU, sL, oL, rs, X64_.get());
     } else if (uDouble) {
         k::leaves<float, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
-                                         (const double*)dU, sL, oL, rs, X32_.get());
+                                         (const double*)dU, sL, oL, rs, X32_.get(),
+                                         !structured_level0(nReal, nPad));
     } else {
         k::leaves<float, float, float>(s, n_, m_, kq, na_, off, nReal, nPad, BbT32_.get(),
-                                       (const float*)dU, sL, oL, rs, X32_.get());
+                                       (const float*)dU, sL, oL, rs, X32_.get(),
+                                       !structured_level0(nReal, nPad));
     }
 }
 
@@ -201,7 +217,7 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
     } else if ((level_kernel() & 1) == 0 && PTm_.get()) {
         k::level_mfma(s, down, level_is_delta(l), na_, d, np,
-                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get());
+                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(), 0, 0, Sg_.get());
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
                                X32_.get());
@@ -225,7 +241,13 @@ int ScanCore::top_level(long nReal, long nPad) const
 void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
-    for (int l = 0; l < lt; ++l) level(s, false, l, pairs_at(nReal, l));
+    int l = 0;
+    if (lt > 0 && structured_level0(nReal, nPad)) {
+        k::level_mfma(s, false, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(),
+                      (3 * n_) / 32 * 32, 3 * n_, Sg_.get());
+        l = 1;
+    }
+    for (; l < lt; ++l) level(s, false, l, pairs_at(nReal, l));
 }
 
 void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSeed, double* dTot)
@@ -250,7 +272,11 @@ void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSe
 void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
-    for (int l = lt - 1; l >= 0; --l) level(s, true, l, pairs_at(nReal, l));
+    const bool s0 = lt > 0 && structured_level0(nReal, nPad);
+    for (int l = lt - 1; l >= (s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));
+    if (s0)   // the X[L] leaves of level 0 hold only their last n rows
+        k::level_mfma(s, true, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(), 0,
+                      3 * n_, Sg_.get());
 }
 
 void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot,

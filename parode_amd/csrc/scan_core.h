@@ -85,6 +85,8 @@ public:
     const double* powerT(int l) const { return PT64_.get() + (size_t)l * na_ * na_; }
     void* X() const { return xIsDouble() ? (void*)X64_.get() : (void*)X32_.get(); }
 
+    bool structured_level0(long nReal, long nPad) const;
+
 private:
     bool level_is_f64(int l) const;
     bool level_is_delta(int l) const;
@@ -106,6 +108,7 @@ private:
     DevArray<double> PT64_;
     DevArray<float> PT32_;
     DevArray<float> PTm_;            // zero-padded FP32 tables of the MFMA levels
+    DevArray<float> Sg_;             // X[R] copy for row-split down-sweep levels (na > 256)
     int nap_ = 0, nrp_ = 0;
     // trajectory
     DevArray<float> X32_;

@@ -273,11 +273,13 @@ def test_precision_modes_against_fp64(solver, mode, bound):
 
 
 # ------------------------------------------- matrix-core vs vector kernels
-@pytest.mark.parametrize("solver,n", [(0, 20), (0, 48), (0, 100), (0, 256), (1, 17), (1, 64),
-                                      (1, 130)])
+@pytest.mark.parametrize("solver,n", [(0, 20), (0, 48), (0, 100), (0, 256), (0, 300), (1, 17),
+                                      (1, 40), (1, 64), (1, 130), (1, 256)])
 def test_mfma_levels_match_vector_levels(solver, n):
-    """The FP32 level update on MFMA (17 <= n_a <= 1024) and on the vector ALU
-    give the same trajectory to FP32 rounding, and both meet the parity bar."""
+    """The FP32 level update on MFMA (17 <= n_a <= 1024; rows split over
+    work-groups above 256; ABM level 0 on the leaves' last block) and on the
+    vector ALU give the same trajectory to FP32 rounding, and both meet the
+    parity bar."""
     N = 2048 if n < 200 else 512
     s = synth.make_system(solver, n, 3, 3, -6.0, -0.2, 1e-3, N, seed=100 + n)
     idx = _register(synth.ufunc_source(s.ucoef))
