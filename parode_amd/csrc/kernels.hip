@@ -1150,9 +1150,14 @@ void carry(hipStream_t s, int na, const double* P, const double* seed, const dou
 }
 
 // ----------------------------------------------------------------- outputs
+// A work-group takes epb elements at a time: their state blocks (n values at
+// offset oX*n of each element) and input rows are loaded coalesced into LDS,
+// with C^T and D^T staged once; then 2^tpeLog threads per element form its
+// outputs from LDS.  The sum runs over c in order, without contraction (the
+// reference's SystemOutput arithmetic in the plain path).
 template <typename TX, typename TU>
 __global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, int oX, bool zC,
-                                                 bool zD, long N, int tpeLog,
+                                                 bool zD, long N, int tpeLog, int epb,
                                                  const float* __restrict__ CT,
                                                  const float* __restrict__ DT,
                                                  const TX* __restrict__ X,
@@ -1160,8 +1165,73 @@ __global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, in
                                                  float* __restrict__ Y)
 {
 #pragma clang fp contract(off)
-    // CT = C^T ([n][p]) and DT = D^T ([m][p]): the threads of one element read
-    // one contiguous row of each per term.  2^tpeLog threads per element.
+    using TA = decltype(TX(0) * TU(0));
+    extern __shared__ unsigned char out_smem[];
+    const int nOut = (zC && zD) ? n : p;
+    const int useC = zC ? 0 : 1, useD = (zD || m <= 0) ? 0 : 1;
+    const int xp = n + 1;                                   // odd pitch
+    // [epb][xp] states, [epb][m] inputs, [n][p] C^T, [m][p] D^T; 16-byte aligned parts
+    size_t off = 0;
+    TX* xs = reinterpret_cast<TX*>(out_smem);
+    off += ((size_t)epb * xp * sizeof(TX) + 15) & ~(size_t)15;
+    TU* us = reinterpret_cast<TU*>(out_smem + off);
+    off += ((size_t)epb * m * useD * sizeof(TU) + 15) & ~(size_t)15;
+    float* cs = reinterpret_cast<float*>(out_smem + off);
+    float* ds = cs + (size_t)n * p * useC;
+    for (int t = threadIdx.x; t < n * p * useC; t += 256) cs[t] = CT[t];
+    for (int t = threadIdx.x; t < m * p * useD; t += 256) ds[t] = DT[t];
+    const int tpe = 1 << tpeLog;
+    const int le = threadIdx.x >> tpeLog, lane = threadIdx.x & (tpe - 1);
+    for (long e0 = (long)blockIdx.x * epb; e0 < N; e0 += (long)gridDim.x * epb) {
+        const int cnt = (N - e0) < epb ? (int)(N - e0) : epb;
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt * n; t += 256) {
+            const int q = t / n, c = t - q * n;
+            xs[q * xp + c] = X[(e0 + q) * na + (long)oX * n + c];
+        }
+        for (int t = threadIdx.x; t < cnt * m * useD; t += 256) {
+            const int q = t / m, c = t - q * m;
+            us[t] = U[((e0 + q) * sO + oO) * m + c];
+        }
+        __syncthreads();
+        if (le >= cnt) continue;
+        const TX* x = xs + le * xp;
+        float* y = Y + (e0 + le) * nOut;
+        for (int j = lane; j < nOut; j += tpe) {
+            if (zC && zD) {
+                y[j] = (float)x[j];
+                continue;
+            }
+            TA v = 0;
+            if (useC)
+                for (int c = 0; c < n; ++c) {
+                    TA pr = (TA)cs[c * p + j] * (TA)x[c];
+                    v = v + pr;
+                }
+            if (useD) {
+                const TU* u = us + le * m;
+                for (int c = 0; c < m; ++c) {
+                    TA pr = (TA)ds[c * p + j] * (TA)u[c];
+                    v = v + pr;
+                }
+            }
+            y[j] = (float)v;
+        }
+    }
+}
+
+// The same outputs with C^T and D^T read from global memory (and L2), for
+// output matrices too large to stage in LDS.
+template <typename TX, typename TU>
+__global__ __launch_bounds__(256) void k_outputs_direct(int n, int m, int p, int na, int oX,
+                                                        bool zC, bool zD, long N, int tpeLog,
+                                                        const float* __restrict__ CT,
+                                                        const float* __restrict__ DT,
+                                                        const TX* __restrict__ X,
+                                                        const TU* __restrict__ U, long sO,
+                                                        long oO, float* __restrict__ Y)
+{
+#pragma clang fp contract(off)
     using TA = decltype(TX(0) * TU(0));
     const int nOut = (zC && zD) ? n : p;
     const int tpe = 1 << tpeLog;
@@ -1202,10 +1272,26 @@ void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool z
     if (N <= 0 || nOut <= 0) return;
     int tpeLog = 0;
     while ((1 << tpeLog) < nOut && tpeLog < 8) ++tpeLog;
-    const long epb = 256 >> tpeLog;
+    const bool useC = !zC, useD = !zD && m > 0;
+    // elements per pass: one per thread group, as many as 48 KiB of LDS holds
+    const size_t fixed = (useC ? (size_t)n * p * 4 : 0) + (useD ? (size_t)m * p * 4 : 0);
+    const size_t per = (size_t)(n + 1) * sizeof(TX) + (useD ? (size_t)m * sizeof(TU) : 0);
+    long epb = 256 >> tpeLog;
+    const size_t budget = 48 * 1024;
+    if (fixed + ((256 >> tpeLog) < 8 ? (256 >> tpeLog) : 8) * per + 32 > budget) {
+        const long e2 = 256 >> tpeLog;
+        const long b2 = std::min<long>((N + e2 - 1) / e2, 1L << 16);
+        hipLaunchKernelGGL((k_outputs_direct<TX, TU>), dim3((unsigned)std::max<long>(b2, 1)),
+                           dim3(256), 0, s, n, m, p, na, oX, zC, zD, N, tpeLog, C, D, X, U, sO,
+                           oO, Y);
+        check_launch("outputs_direct");
+        return;
+    }
+    if (fixed + epb * per + 32 > budget) epb = (long)((budget - fixed - 32) / per);
+    const size_t lds = fixed + (size_t)epb * per + 32;   // + alignment of the parts
     const long blocks = std::min<long>((N + epb - 1) / epb, 1L << 16);
-    hipLaunchKernelGGL((k_outputs<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256), 0,
-                       s, n, m, p, na, oX, zC, zD, N, tpeLog, C, D, X, U, sO, oO, Y);
+    hipLaunchKernelGGL((k_outputs<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
+                       lds, s, n, m, p, na, oX, zC, zD, N, tpeLog, (int)epb, C, D, X, U, sO, oO, Y);
     check_launch("outputs");
 }
 

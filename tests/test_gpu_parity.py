@@ -318,3 +318,13 @@ def test_each_kernel_choice_meets_parity(solver, variant):
     err = orc.max_rel_err(y, ye)
     print(f"variant {variant}: {err:.2e}")
     assert err <= PARITY
+
+
+@pytest.mark.parametrize("solver,n,p", [(0, 200, 64), (1, 40, 96)])
+def test_large_output_matrices(solver, n, p):
+    """Output matrices too large to stage in LDS take the global-memory
+    output kernel; parity as everywhere."""
+    s = synth.make_system(solver, n, 3, p, -6.0, -0.2, 1e-3, 700, seed=300 + n)
+    tv, y = _run_ufunc(solver, s)
+    assert y.shape == (700, p)
+    _check_plain(solver, s, y, tv)
