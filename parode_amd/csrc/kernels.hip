@@ -712,7 +712,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // X[R]).  The down-sweep reads its source columns from S, a compact copy of
 // the old X[R] ([np][na], made by k_gather_r), because other row blocks write
 // X[R] while this one still reads it.  S is null when one block covers n_a.
-template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC>
+template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, long d, long np,
                                                     int jbeg, int rbeg,
                                                     const float* __restrict__ PT,
@@ -729,8 +729,8 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     const int wi = wave % WGI, wc = wave / WGI;
     const int l32 = lane & 31, hi = lane >> 5;
     const long k0 = (long)blockIdx.x * TC;
-    const int ib0 = blockIdx.y * IB;
-    const bool fromS = DOWN && S != nullptr;
+    const int ib0 = SPLIT ? blockIdx.y * IB : 0;
+    constexpr bool fromS = DOWN && SPLIT;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
     floatx16 acc[NCT][NIT];
 #pragma unroll
@@ -856,40 +856,54 @@ __global__ void k_gather_r(int na, long d, long np, const float* __restrict__ X,
     }
 }
 
+template <int NIT, int NCT, int WGI, bool SPLIT>
+static void mfma_launch_t(hipStream_t s, bool down, bool delta, int na, int nap, int nrp, long d,
+                          long np, int jbeg, int rbeg, const float* PT, float* X, const float* S,
+                          dim3 g)
+{
+    constexpr int TC = (4 / WGI) * NCT * 32;
+    constexpr int KC = 8192 / TC;
+    dim3 b(256);
+    if (down) {
+        if (delta)
+            hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s, na,
+                               nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+        else
+            hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
+                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+    } else {
+        if (delta)
+            hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
+                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+        else
+            hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
+                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+    }
+    check_launch("level_mfma");
+}
+
 template <int NIT, int NCT, int WGI>
 static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
                         int jbeg, int rbeg, const float* PT, float* X, float* S)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
-    constexpr int KC = 8192 / TC;
     constexpr int IB = WGI * NIT * 32;
     const int nap = (na + 31) / 32 * 32, nrp = mfma_table_rows(na);
     const unsigned gy = (unsigned)((na + IB - 1) / IB);
-    if (gy > 1 && down) {
+    dim3 g((unsigned)((np + TC - 1) / TC), gy);
+    if (gy == 1) {
+        mfma_launch_t<NIT, NCT, WGI, false>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, PT, X,
+                                            nullptr, g);
+        return;
+    }
+    if (down) {
         if (!S) throw std::invalid_argument("level_mfma: split rows need the gather buffer");
         hipLaunchKernelGGL(k_gather_r, dim3(blocks_for(np * na, 256, 8192)), dim3(256), 0, s, na,
                            d, np, X, S);
         check_launch("gather_r");
-    } else {
-        S = nullptr;
     }
-    dim3 g((unsigned)((np + TC - 1) / TC), gy), b(256);
-    if (down) {
-        if (delta)
-            hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap, nrp, d,
-                               np, jbeg, rbeg, PT, S, X);
-        else
-            hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, jbeg, rbeg, PT, S, X);
-    } else {
-        if (delta)
-            hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, jbeg, rbeg, PT, S, X);
-        else
-            hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC>), g, b, 0, s, na, nap,
-                               nrp, d, np, jbeg, rbeg, PT, S, X);
-    }
-    check_launch("level_mfma");
+    mfma_launch_t<NIT, NCT, WGI, true>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, PT, X, S,
+                                       g);
 }
 
 // Rows of the padded MFMA table: a multiple of every K-slice (<= 256).
