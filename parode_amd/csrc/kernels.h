@@ -45,8 +45,9 @@ void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, do
 // X[e][off + r] = sum_{q<k} Bbar[r][q*m + j] * U[(e*sL + oL + q*rs)*m + j], e < nReal;
 // rows off..off+n of X for e in [nReal, nPad) are zero; other rows of X are
 // zeroed when zeroOther (the ABM zero fill).  T: X type, TU: U type.
-// writeZeros = false leaves the rows outside [off, off + n) unwritten (for a
-// first level that never reads them).
+// writeZeros = false writes, of the rows outside [off, off + n), only rows
+// [32 floor(off / 32), off) of real elements (all of them for padding
+// elements): what a first level that starts at state 32 floor(off / 32) reads.
 template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, long nPad,
             const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X, bool writeZeros = true);

@@ -471,7 +471,7 @@ constexpr int kLeafEB = 8;
 
 template <typename T, typename TU, typename TB>
 __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
-                                                long nPad, int tpeLog, bool writeZeros,
+                                                long nPad, int tpeLog, bool writeZeros, int zlo,
                                                 const TB* __restrict__ BbT,
                                                 const TU* __restrict__ U, long sL, long oL,
                                                 long rs, T* __restrict__ X)
@@ -495,13 +495,20 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
             const long e = eb + le;
             Us[t] = e < nReal ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
         }
-        // zero rows [0, off) and [off + n, na) of the block's elements (of its
-        // padding elements only, when the first level does not read them)
+        // zero rows [0, off) and [off + n, na) of the block's elements; when the
+        // first level reads only rows from zlo on, rows [zlo, off) of real
+        // elements (and every zero row of padding elements)
         const long firstZ = writeZeros ? eb : (nReal > eb ? nReal : eb);
         const int z0 = firstZ - eb < cnt ? (int)(firstZ - eb) * nz : cnt * nz;
         for (int t = z0 + threadIdx.x; t < cnt * nz; t += 256) {
             const int le = t / nz, z = t - le * nz;
             X[(eb + le) * na + (z < off ? z : z + n)] = (T)0;
+        }
+        const int nzl = off - zlo;                 // rows [zlo, off) of real elements
+        const int creal = writeZeros ? 0 : (int)((firstZ - eb) < cnt ? firstZ - eb : cnt);
+        for (int t = threadIdx.x; t < creal * nzl; t += 256) {
+            const int le = t / nzl, z = t - le * nzl;
+            X[(eb + le) * na + zlo + z] = (T)0;
         }
         __syncthreads();
         const long e0 = eb + (long)grp * kLeafEB;
@@ -543,8 +550,8 @@ void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, lo
     const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
     const size_t lds = (size_t)perBlock * kq * m * sizeof(TA);
     hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
-                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, writeZeros, Bbar, U, sL,
-                       oL, rs, X);
+                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, writeZeros, off / 32 * 32,
+                       Bbar, U, sL, oL, rs, X);
     check_launch("leaves");
 }
 
@@ -752,10 +759,14 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
         for (int u = 0; u < PER; ++u) {
             const int t = threadIdx.x + u * 256;
             const int c = t / KC, j = t % KC;
-            const long R = (k0 + c + 1) * 2 * d - 1;
-            v[u] = (c < ncol && j < kc && (DOWN || j0 + j >= rbeg))
-                       ? (fromS ? S[(k0 + c) * na + j0 + j] : X[(DOWN ? R : R - d) * na + j0 + j])
-                       : 0.0f;
+            // Clamped indices, so every load is in bounds and none is
+            // predicated: columns past the tile repeat its last column (their
+            // products are never stored), states past na repeat the last
+            // state (they meet the zero rows of the padded table), and the
+            // level-0 leaves hold zeros from jbeg on.
+            const int cc = c < ncol ? c : ncol - 1, jj = j < kc ? j : kc - 1;
+            const long R = (k0 + cc + 1) * 2 * d - 1;
+            v[u] = fromS ? S[(k0 + cc) * na + j0 + jj] : X[(DOWN ? R : R - d) * na + j0 + jj];
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -813,16 +824,30 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
             const int i = ib0 + (wi + it * WGI) * 32 + l32;
             if (i >= na) continue;
             float xr[16], xl[16];
+            // rows below rbeg of a level-0 leaf are zero by definition (and
+            // not stored): one branch per tile row, the loads inside it
+            // unpredicated (columns clamped as in the staging)
+            const bool liveR = DOWN || i >= rbeg, liveL = (DOWN || DELTA) && (!DOWN || i >= rbeg);
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                const long R = (k0 + cr + 1) * 2 * d - 1;
-                const bool ok = full || cr < ncol;
-                xr[v] = (ok && (DOWN || i >= rbeg))
-                            ? (fromS ? S[(k0 + cr) * na + i] : X[R * na + i])
-                            : 0.0f;
-                xl[v] = (ok && (DOWN || DELTA) && (!DOWN || i >= rbeg)) ? X[(R - d) * na + i]
-                                                                       : 0.0f;
+                xr[v] = 0.0f;
+                xl[v] = 0.0f;
+            }
+            if (liveR) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                    const int crc = cr < ncol ? cr : ncol - 1;
+                    xr[v] = fromS ? S[(k0 + crc) * na + i] : X[((k0 + crc + 1) * 2 * d - 1) * na + i];
+                }
+            }
+            if (liveL) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                    const int crc = cr < ncol ? cr : ncol - 1;
+                    xl[v] = X[((k0 + crc + 1) * 2 * d - 1 - d) * na + i];
+                }
             }
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
