@@ -739,13 +739,35 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     const int ib0 = SPLIT ? blockIdx.y * IB : 0;
     constexpr bool fromS = DOWN && SPLIT;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
+    const bool full = ncol == TC;
+    // Up-sweep: the accumulators start from the addend X[R] (plus X[L] in the
+    // delta form), loaded before the first K-slice so their latency hides
+    // behind it, and the epilogue only stores.  Down-sweep: they start at zero
+    // and the epilogue loads X[R] and X[L] (a preload would raise the register
+    // count past two waves per SIMD).  Columns past the tile are clamped
+    // (loaded, never stored); level-0 leaf rows below rbeg are zero (one
+    // branch per tile row).
     floatx16 acc[NCT][NIT];
 #pragma unroll
-    for (int a = 0; a < NCT; ++a)
+    for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
-        for (int b = 0; b < NIT; ++b)
+        for (int it = 0; it < NIT; ++it) {
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+            for (int v = 0; v < 16; ++v) acc[ct][it][v] = 0.0f;
+            const int i = ib0 + (wi + it * WGI) * 32 + l32;
+            const int ic = i < na ? i : na - 1;
+            if (!DOWN && i >= rbeg) {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                    const int crc = cr < ncol ? cr : ncol - 1;
+                    const long R = (k0 + crc + 1) * 2 * d - 1;
+                    float a = X[R * na + ic];
+                    if (DELTA) a += X[(R - d) * na + ic];
+                    acc[ct][it][v] = a;
+                }
+            }
+        }
     const float* pcol[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -816,32 +838,31 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
         }
         __syncthreads();
     }
-    const bool full = ncol == TC;
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int i = ib0 + (wi + it * WGI) * 32 + l32;
             if (i >= na) continue;
-            float xr[16], xl[16];
-            // rows below rbeg of a level-0 leaf are zero by definition (and
-            // not stored): one branch per tile row, the loads inside it
-            // unpredicated (columns clamped as in the staging)
-            const bool liveR = DOWN || i >= rbeg, liveL = (DOWN || DELTA) && (!DOWN || i >= rbeg);
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                xr[v] = 0.0f;
-                xl[v] = 0.0f;
-            }
-            if (liveR) {
+            if (!DOWN) {
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                    const int crc = cr < ncol ? cr : ncol - 1;
-                    xr[v] = fromS ? S[(k0 + crc) * na + i] : X[((k0 + crc + 1) * 2 * d - 1) * na + i];
+                    if (!full && cr >= ncol) continue;
+                    X[((k0 + cr + 1) * 2 * d - 1) * na + i] = acc[ct][it][v];
                 }
+                continue;
             }
-            if (liveL) {
+            // down: X[L] <- old X[R]; X[R] <- P old X[R] + old X[L]
+            float xr[16], xl[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                const int crc = cr < ncol ? cr : ncol - 1;
+                xr[v] = fromS ? S[(k0 + crc) * na + i] : X[((k0 + crc + 1) * 2 * d - 1) * na + i];
+                xl[v] = 0.0f;
+            }
+            if (i >= rbeg) {
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
@@ -855,14 +876,9 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
                 if (!full && cr >= ncol) continue;
                 const long R = (k0 + cr + 1) * 2 * d - 1;
                 float y = acc[ct][it][v];
-                if (!DOWN) {
-                    if (DELTA) y = y + xl[v];
-                    X[R * na + i] = xr[v] + y;
-                } else {
-                    if (DELTA) y = y + xr[v];
-                    X[(R - d) * na + i] = xr[v];
-                    X[R * na + i] = y + xl[v];
-                }
+                if (DELTA) y = y + xr[v];
+                X[(R - d) * na + i] = xr[v];
+                X[R * na + i] = y + xl[v];
             }
         }
 }
