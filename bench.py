@@ -31,12 +31,18 @@ PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 TRAFFIC_LEVEL0_UP = (529458 * 2.0 + 524287) * 1024
 
 
-def level_flops(na, n_scan):
+def level_flops(na, n_scan, n_leaf=None):
     """Algorithmic flops of the level sweeps: 2 na^2 per pair, pairs of
-    level l = ceil(N / 2^(l+1)), both sweeps."""
+    level l = ceil(N / 2^(l+1)), both sweeps.  With n_leaf (ABM: the n
+    nonzero rows of each n_a-row leaf), level 0 of the up-sweep costs
+    2 na n_leaf per pair: only those source states are nonzero."""
     L = max(0, (n_scan - 1).bit_length())
-    pairs = sum((n_scan + (2 << l) - 1) // (2 << l) for l in range(L))
-    return 2 * 2.0 * na * na * pairs, pairs
+    per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
+    pairs = sum(per)
+    flops = 2 * 2.0 * na * na * pairs
+    if n_leaf and L:
+        flops -= 2.0 * na * (na - n_leaf) * per[0]
+    return flops, pairs
 
 
 def cpu_baseline_leg(s, U64, y_gpu, target_s=12.0, full_reference=True):
@@ -312,7 +318,7 @@ def main():
 
     na = s.n_scan
     n_scan = s.steps if s.method == 0 else s.steps - 3
-    flops, pairs = level_flops(na, n_scan)
+    flops, pairs = level_flops(na, n_scan, s.n if s.method == 1 else None)
     sweep_ms = ph["up_sweep"] + ph["down_sweep"]
     achieved = flops / (sweep_ms * 1e-3)
     peak = PEAK["fp64"] if prec == api.FP64 else PEAK["fp32"]
