@@ -74,6 +74,15 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
                 float* X, int jbeg = 0, int rbeg = 0, float* S = nullptr);
 inline bool level_mfma_splits(int na) { return na > 256; }
 
+// A level with few pairs as many small K-split tiles (32 x 32 FP32 MFMA when
+// TT = float, with PT the padded MFMA table and ldp its row length; 16 x 16
+// FP64 MFMA when TT = double, with PT the transposed FP64 power, ldp = na).
+// The down-sweep first copies the pairs' X[R] into S (np * na values).
+constexpr long kKsMaxPairs = 256;
+template <typename TT, typename TX>
+void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
+              TX* S);
+
 // Levels [l0, L) of both sweeps and the root in one work-group (levels >= l64
 // from the FP64 table, the others from the FP32 one).
 template <typename TX>

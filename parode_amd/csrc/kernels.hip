@@ -981,6 +981,154 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
     }
 }
 
+// ------------------------------------------- levels with few pairs (K split)
+// A work-group computes one tile of pairs x rows (32 x 32 on FP32 MFMA,
+// 16 x 16 on FP64 MFMA); its four waves split the K range (the source states)
+// and their partial tiles meet in LDS.  The many small tiles spread a level
+// with few pairs over many CUs, where one work-group per column block would
+// stream the whole table through one CU.  Sources: X[L] (up) or S, a copy of
+// the old X[R] (down; other tiles write X[R] rows meanwhile).  Tables: the
+// padded transposed FP32 table [nrp][nap] or the transposed FP64 power
+// [na][na].  Up: X[R] += P X[L]; down: X[L] = old X[R], X[R] = P old X[R] +
+// X[L].  Loads are clamped into range and issued eight k-steps ahead.
+template <typename TT>
+struct KsTile;
+template <>
+struct KsTile<float> {
+    static constexpr int T = 32, KS = 2, NACC = 16;
+    using acc_t = floatx16;
+};
+template <>
+struct KsTile<double> {
+    static constexpr int T = 16, KS = 4, NACC = 4;
+    using acc_t = doublex4;
+};
+
+template <typename TT, typename TX, bool DOWN>
+__global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
+                                                  const TT* __restrict__ PT,
+                                                  const TX* __restrict__ S, TX* __restrict__ X)
+{
+    using K = KsTile<TT>;
+    constexpr int T = K::T, KS = K::KS, NACC = K::NACC, U = 8;
+    __shared__ TT part[3][NACC][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long c0 = (long)blockIdx.x * T;
+    const int i0 = blockIdx.y * T;
+    // fragment coordinates: FP32 A(c = l%32, k = l/32), B(k = l/32, i = l%32);
+    // FP64 A(c = l%16, k = l/16), B(k = l/16, i = l%16)
+    const int lc = lane % T, lk = lane / T;
+    const long colA = (c0 + lc) < np ? c0 + lc : np - 1;
+    const TX* src = DOWN ? S + colA * na : X + ((colA + 1) * 2 * d - 1 - d) * na;
+    const int ib = i0 + lc;
+    const int ibc = (sizeof(TT) == 4 || ib < na) ? ib : na - 1;   // FP32 table is padded
+    const int ksteps = (na + KS - 1) / KS, per = (ksteps + 3) / 4;
+    const int kb = wave * per, ke = (kb + per) < ksteps ? kb + per : ksteps;
+    typename K::acc_t acc;
+#pragma unroll
+    for (int v = 0; v < NACC; ++v) acc[v] = 0;
+    for (int k = kb; k < ke; k += U) {
+        TT a[U], b[U];
+        TX araw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // clamped, unpredicated loads, all issued before the MFMAs; a
+            // k-step past the range or a state past na contributes a zero A
+            // value (the loaded values are finite)
+            const int j = (k + u) * KS + lk;
+            const int jc = j < na ? j : na - 1;
+            araw[u] = src[jc];
+            b[u] = PT[(long)jc * ldp + ibc];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = (k + u) * KS + lk;
+            a[u] = (k + u < ke && j < na) ? (TT)araw[u] : (TT)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if constexpr (sizeof(TT) == 4)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+            else
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int v = 0; v < NACC; ++v) part[wave - 1][v][lane] = acc[v];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    // D: FP32 (c = 8(v/4) + 4(l/32) + v%4, i = l%32); FP64 (c = 4v + l/16, i = l%16).
+    // All loads first (clamped addresses), then the guarded stores.
+    const int i = i0 + lc, icl = i < na ? i : na - 1;
+    TX xo[NACC], xs[NACC];
+#pragma unroll
+    for (int v = 0; v < NACC; ++v) {
+        const int c = (sizeof(TT) == 4) ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
+                                        : 4 * v + (lane >> 4);
+        const long col = (c0 + c) < np ? c0 + c : np - 1;
+        const long R = (col + 1) * 2 * d - 1;
+        xo[v] = X[(DOWN ? R - d : R) * na + icl];
+        xs[v] = DOWN ? S[col * na + icl] : (TX)0;
+    }
+#pragma unroll
+    for (int v = 0; v < NACC; ++v) {
+        const TT sum = acc[v] + part[0][v][lane] + part[1][v][lane] + part[2][v][lane];
+        const int c = (sizeof(TT) == 4) ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
+                                        : 4 * v + (lane >> 4);
+        const long col = c0 + c;
+        if (col >= np || i >= na) continue;
+        const long R = (col + 1) * 2 * d - 1;
+        if (!DOWN) {
+            X[R * na + i] = (TX)((TT)xo[v] + sum);
+        } else {
+            X[(R - d) * na + i] = xs[v];
+            X[R * na + i] = (TX)(sum + (TT)xo[v]);
+        }
+    }
+}
+
+template <typename TX>
+__global__ void k_gather_rt(int na, long d, long np, const TX* __restrict__ X, TX* __restrict__ S)
+{
+    const long total = np * na;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long c = e / na;
+        const int j = (int)(e - c * na);
+        S[e] = X[((c + 1) * 2 * d - 1) * na + j];
+    }
+}
+
+template <typename TT, typename TX>
+void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
+              TX* S)
+{
+    if (np <= 0) return;
+    constexpr int T = KsTile<TT>::T;
+    if (sizeof(TT) == 4 && ldp % 32 != 0)
+        throw std::invalid_argument("level_ks: FP32 table must be the padded MFMA table");
+    dim3 g((unsigned)((np + T - 1) / T), (unsigned)((na + T - 1) / T)), b(256);
+    if (down) {
+        if (!S) throw std::invalid_argument("level_ks: the down-sweep needs the copy buffer");
+        hipLaunchKernelGGL((k_gather_rt<TX>), dim3(blocks_for(np * na, 256, 4096)), dim3(256), 0, s,
+                           na, d, np, X, S);
+        hipLaunchKernelGGL((k_level_ks<TT, TX, true>), g, b, 0, s, na, ldp, d, np, PT, S, X);
+    } else {
+        hipLaunchKernelGGL((k_level_ks<TT, TX, false>), g, b, 0, s, na, ldp, d, np, PT, S, X);
+    }
+    check_launch("level_ks");
+}
+
+template void level_ks<float, float>(hipStream_t, bool, int, int, long, long, const float*, float*,
+                                     float*);
+template void level_ks<double, float>(hipStream_t, bool, int, int, long, long, const double*,
+                                      float*, float*);
+template void level_ks<double, double>(hipStream_t, bool, int, int, long, long, const double*,
+                                       double*, double*);
+
 // ------------------------------------------------------- top of the tree
 // Levels [l0, L) of the up-sweep, the root and the same levels of the
 // down-sweep in ONE work-group of 1024 threads: the top of the tree holds few

@@ -1,5 +1,7 @@
 #include "scan_core.h"
 
+#include <algorithm>
+
 #include "config.h"
 #include "kernels.h"
 
@@ -43,6 +45,14 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
         X64_.alloc((size_t)nPadMax_ * na_, arena);
     else
         X32_.alloc((size_t)nPadMax_ * na_, arena);
+    // down-sweep copy of X[R] for the K-split levels with few pairs
+    if (na_ >= kKsMinStates) {
+        const size_t ks = (size_t)std::min<long>(k::kKsMaxPairs, nPadMax_ / 2 > 0 ? nPadMax_ / 2 : 1) * na_;
+        if (xIsDouble())
+            Sks64_.alloc(ks, arena);
+        else
+            Sks32_.alloc(ks, arena);
+    }
     seed_.alloc(na_, arena);
     tot_.alloc(na_, arena);
     seedNext_.alloc(na_, arena);
@@ -64,6 +74,9 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
         if (k::level_mfma_splits((int)na)) b += 4 * (size_t)(nP / 2 > 0 ? nP / 2 : 1) * na;
     }
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
+    if ((int)na >= kKsMinStates)
+        b += (opt.precision == kFp64 ? 8 : 4) *
+             (size_t)std::min<long>(k::kKsMaxPairs, nP / 2 > 0 ? nP / 2 : 1) * na;
     b += 24 * na;
     return b + 20 * Arena::kAlign;   // block rounding when carved from an Arena
 }
@@ -72,8 +85,8 @@ size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
            Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + CT_.bytes() + DT_.bytes() +
-           PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + Sg_.bytes() + X32_.bytes() + X64_.bytes() +
-           seed_.bytes() +
+           PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + Sg_.bytes() + Sks32_.bytes() +
+           Sks64_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
            tot_.bytes() + seedNext_.bytes();
 }
 
@@ -211,6 +224,25 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
 {
     const size_t na2 = (size_t)na_ * na_;
     const long d = 1L << l;
+    // few pairs: small K-split tiles on the matrix cores, spread over many CUs
+    if (np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l)) {
+        if (xIsDouble()) {
+            k::level_ks<double, double>(s, down, na_, na_, d, np, PT64_.get() + l * na2,
+                                        X64_.get(), Sks64_.get());
+            return;
+        }
+        if (level_is_f64(l)) {
+            k::level_ks<double, float>(s, down, na_, na_, d, np, PT64_.get() + l * na2,
+                                       X32_.get(), Sks32_.get());
+            return;
+        }
+        if ((level_kernel() & 1) == 0 && PTm_.get()) {
+            k::level_ks<float, float>(s, down, na_, nap_, d, np,
+                                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(),
+                                      Sks32_.get());
+            return;
+        }
+    }
     if (xIsDouble()) {
         k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
     } else if (level_is_f64(l)) {
@@ -233,6 +265,9 @@ int ScanCore::top_level(long nReal, long nPad) const
     const int L = ilog2(nPad);
     if (level_kernel() & 4) return L;                // fused top switched off
     if ((long)nPad * na_ <= 65536) return 0;         // small: the whole tree in one work-group
+    // from 64 states on, the levels with few pairs run as K-split launches
+    // over many CUs, faster than streaming each table through one CU
+    if (na_ >= 64) return L;
     int l = 0;
     while (l < L && pairs_at(nReal, l) > 32) ++l;
     return l;

@@ -21,6 +21,7 @@
 namespace parode {
 
 enum Method { kRK4 = 0, kABM = 1 };
+constexpr int kKsMinStates = 16;   // smallest scan size for the K-split levels
 enum Precision { kPlain32 = 0, kMixed = 1, kFp64 = 2 };
 
 struct ScanOptions {
@@ -109,6 +110,8 @@ private:
     DevArray<float> PT32_;
     DevArray<float> PTm_;            // zero-padded FP32 tables of the MFMA levels
     DevArray<float> Sg_;             // X[R] copy for row-split down-sweep levels (na > 256)
+    DevArray<float> Sks32_;          // X[R] copy for K-split down-sweep levels (few pairs)
+    DevArray<double> Sks64_;
     int nap_ = 0, nrp_ = 0;
     // trajectory
     DevArray<float> X32_;
