@@ -41,6 +41,13 @@ void abm_combine(hipStream_t s, bool mirror, int n, int m, double h, const void*
 void make_level_table_f32(hipStream_t s, int na, const double* P, bool delta, float* PT);
 void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, double* PT);
 
+// All FP32 level tables [0, nlev) in one launch (PT32 and/or the padded PTm).
+void make_level_tables(hipStream_t s, int na, int nap, int nrp, int nlev, int deltaBelow,
+                       const double* PT64, float* PT32, float* PTm);
+// PT[l + 1] = PT[l]^2 for l < L in one work-group (n_a <= 80), FP64 MFMA.
+bool powers_small_supported(int na);
+void powers_small(hipStream_t s, int na, int L, double* PT);
+
 // ------------------------------------------------------------------- leaves
 // X[e][off + r] = sum_{q<k} Bbar[r][q*m + j] * U[(e*sL + oL + q*rs)*m + j], e < nReal;
 // rows off..off+n of X for e in [nReal, nPad) are zero; other rows of X are

@@ -132,11 +132,16 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int r, int kk, int c,
 
 __global__ void k_gemm_f64_mfma(int r, int kk, int c, const double* __restrict__ X,
                                 const double* __restrict__ Y, double* __restrict__ Z);
+__global__ void k_gemm_f64_tiled(int r, int kk, int c, const double* __restrict__ X,
+                                 const double* __restrict__ Y, double* __restrict__ Z);
 
 void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
               bool mfma)
 {
-    if (mfma) {
+    if (mfma && r >= 256 && c >= 256 && kk >= 64) {
+        dim3 grid((c + 63) / 64, (r + 63) / 64);
+        hipLaunchKernelGGL(k_gemm_f64_tiled, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    } else if (mfma) {
         dim3 grid((c + 15) / 16, (r + 15) / 16);
         hipLaunchKernelGGL(k_gemm_f64_mfma, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
     } else {
@@ -196,6 +201,75 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
             if (i < r && j < c) Z[(long)i * c + j] = sum;
         }
     }
+}
+
+// FP64 GEMM for large operands (the squarings of big power tables): a
+// work-group owns a 64 x 64 tile of Z, each wave a 32 x 32 quarter (2 x 2
+// MFMA tiles); 16-deep K-slices of X and Y pass through LDS, the next slice's
+// loads in flight in registers while the current one is multiplied.
+__global__ __launch_bounds__(256) void k_gemm_f64_tiled(int r, int kk, int c,
+                                                        const double* __restrict__ X,
+                                                        const double* __restrict__ Y,
+                                                        double* __restrict__ Z)
+{
+    constexpr int T = 64, KB = 16, AP = KB + 1, BP = T + 1;
+    __shared__ double As[T * AP];
+    __shared__ double Bs[KB * BP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1, l16 = lane & 15, q4 = lane >> 4;
+    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
+    // staging: A rows ar = tid / 4 (4 consecutive k each), B rows bk = tid / 16 (4 consecutive j)
+    const int ar = tid >> 2, ak = (tid & 3) * 4, bk = tid >> 4, bj = (tid & 15) * 4;
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gi = i0 + ar, gk = k0 + ak + q;
+            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
+            const int gk2 = k0 + bk, gj = j0 + bj + q;
+            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
+        }
+    };
+    doublex4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = {0.0, 0.0, 0.0, 0.0};
+    load(0);
+    for (int k0 = 0; k0 < kk; k0 += KB) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            As[ar * AP + ak + q] = ra[q];
+            Bs[bk * BP + bj + q] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + KB < kk) load(k0 + KB);
+#pragma unroll
+        for (int k4 = 0; k4 < KB; k4 += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = As[(wr * 32 + t * 16 + l16) * AP + k4 + q4];
+                b[t] = Bs[(k4 + q4) * BP + wc * 32 + t * 16 + l16];
+            }
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+    }
+    // D[i][j] is in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wr * 32 + ti * 16 + 4 * v + q4, j = j0 + wc * 32 + tj * 16 + l16;
+                if (i < r && j < c) Z[(long)i * c + j] = acc[ti][tj][v];
+            }
 }
 
 __global__ void k_cast_f32_f64(long count, const float* __restrict__ in, double* __restrict__ out)
@@ -453,6 +527,107 @@ void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, do
     hipLaunchKernelGGL(k_level_table<double>, dim3(blocks_for(nn, 256, 8192)), dim3(256), 0, s, nn,
                        na, P, delta, PT);
     check_launch("level_table_f64");
+}
+
+// All FP32 level tables of a solve in one launch: levels [0, nlev) of the
+// transposed FP64 powers (delta form below deltaBelow) as the unpadded FP32
+// table PT32 [na][na] (if not null) and the zero-padded MFMA table PTm
+// [nrp][nap] (if not null; else nap = nrp = na).
+__global__ void k_level_tables(int na, int nap, int nrp, int nlev, int deltaBelow,
+                               const double* __restrict__ PT64, float* __restrict__ PT32,
+                               float* __restrict__ PTm)
+{
+    const long per = (long)nrp * nap, nn = (long)na * na;
+    const long total = per * nlev;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int l = (int)(idx / per);
+        const long r = idx - l * per;
+        const int j = (int)(r / nap), i = (int)(r - (long)j * nap);
+        float v = 0.0f;
+        if (j < na && i < na) {
+            double x = PT64[l * nn + (long)j * na + i];
+            if (l < deltaBelow && i == j) x -= 1.0;
+            v = (float)x;
+            if (PT32) PT32[l * nn + (long)j * na + i] = v;
+        }
+        if (PTm) PTm[idx] = v;
+    }
+}
+
+void make_level_tables(hipStream_t s, int na, int nap, int nrp, int nlev, int deltaBelow,
+                       const double* PT64, float* PT32, float* PTm)
+{
+    if (nlev <= 0) return;
+    if (!PTm) nap = nrp = na;
+    const long total = (long)nrp * nap * nlev;
+    hipLaunchKernelGGL(k_level_tables, dim3(blocks_for(total, 256, 16384)), dim3(256), 0, s, na,
+                       nap, nrp, nlev, deltaBelow, PT64, PT32, PTm);
+    check_launch("level_tables");
+}
+
+// All squarings of a small power table in one work-group: the transposed
+// FP64 table P_l (n_a <= 80) sits in LDS (pitch odd in doubles); each of the
+// 16 waves forms 16 x 16 tiles of P_l^2 on the FP64 matrix cores, the tiles go
+// to LDS and to PT[l + 1] after a barrier.  Replaces L launches of the GEMM.
+constexpr int kPowersSmallMax = 80;
+
+__global__ __launch_bounds__(1024) void k_powers_small(int na, int L, double* __restrict__ PT)
+{
+    extern __shared__ double Ps[];
+    const int n16 = (na + 15) / 16 * 16, pitch = n16 + 1;
+    const int nt = n16 / 16, ntiles = nt * nt;
+    const long nn = (long)na * na;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, q4 = lane >> 4;
+    for (int t = threadIdx.x; t < n16 * n16; t += 1024) {
+        const int r = t / n16, c = t - r * n16;
+        Ps[r * pitch + c] = (r < na && c < na) ? PT[(long)r * na + c] : 0.0;
+    }
+    __syncthreads();
+    for (int l = 0; l < L; ++l) {
+        doublex4 acc[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            acc[q] = {0.0, 0.0, 0.0, 0.0};
+            const int tile = wave + 16 * q;
+            if (tile >= ntiles) continue;
+            const int ti = tile / nt, tj = tile - ti * nt;
+            for (int k = 0; k < n16; k += 4) {
+                const double a = Ps[(ti * 16 + l16) * pitch + k + q4];
+                const double b = Ps[(k + q4) * pitch + tj * 16 + l16];
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        double* out = PT + (long)(l + 1) * nn;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int tile = wave + 16 * q;
+            if (tile >= ntiles) continue;
+            const int ti = tile / nt, tj = tile - ti * nt;
+            // D[i][j] is in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ti * 16 + 4 * v + q4, c = tj * 16 + l16;
+                Ps[r * pitch + c] = acc[q][v];
+                if (r < na && c < na) out[(long)r * na + c] = acc[q][v];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+bool powers_small_supported(int na) { return na <= kPowersSmallMax; }
+
+void powers_small(hipStream_t s, int na, int L, double* PT)
+{
+    if (L <= 0) return;
+    if (na > kPowersSmallMax) throw std::invalid_argument("powers_small: n_a too large");
+    const int n16 = (na + 15) / 16 * 16;
+    const size_t lds = (size_t)n16 * (n16 + 1) * sizeof(double);
+    hipLaunchKernelGGL(k_powers_small, dim3(1), dim3(1024), lds, s, na, L, PT);
+    check_launch("powers_small");
 }
 
 // ------------------------------------------------------------------ leaves

@@ -171,18 +171,19 @@ void ScanCore::build_powers(hipStream_t s)
     const size_t na2 = (size_t)na_ * na_;
     // (P^T)^2 = (P^2)^T: squaring the transposed table keeps it transposed
     const bool mm = (level_kernel() & 2) == 0;
-    for (int l = 0; l < L_; ++l)
-        k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
-                    PT64_.get() + (l + 1) * na2, mm);
-    for (int l = 0; l < L_; ++l)
-        if (!level_is_f64(l)) {
-            k::make_level_table_f32(s, na_, PT64_.get() + l * na2, level_is_delta(l),
-                                    PT32_.get() + l * na2);
-            if (PTm_.get())
-                k::make_level_table_padded(s, na_, nap_, nrp_, PT64_.get() + l * na2,
-                                           level_is_delta(l),
-                                           PTm_.get() + (size_t)l * nap_ * nrp_);
-        }
+    if (mm && k::powers_small_supported(na_)) {
+        k::powers_small(s, na_, L_, PT64_.get());
+    } else {
+        for (int l = 0; l < L_; ++l)
+            k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
+                        PT64_.get() + (l + 1) * na2, mm);
+    }
+    int n32 = 0, nd = 0;
+    for (int l = 0; l < L_; ++l) {
+        if (!level_is_f64(l)) n32 = l + 1;
+        if (level_is_delta(l)) nd = l + 1;
+    }
+    k::make_level_tables(s, na_, nap_, nrp_, n32, nd, PT64_.get(), PT32_.get(), PTm_.get());
 }
 
 // -------------------------------------------------------------------- leaves
