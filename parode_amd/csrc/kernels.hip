@@ -138,7 +138,7 @@ __global__ void k_gemm_f64_tiled(int r, int kk, int c, const double* __restrict_
 void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
               bool mfma)
 {
-    if (mfma && r >= 256 && c >= 256 && kk >= 64) {
+    if (mfma && r >= 512 && c >= 512 && kk >= 64) {
         dim3 grid((c + 63) / 64, (r + 63) / 64);
         hipLaunchKernelGGL(k_gemm_f64_tiled, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
     } else if (mfma) {

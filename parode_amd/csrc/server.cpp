@@ -83,7 +83,8 @@ Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
     load_.assign(lanes, 0);
     for (auto& s : lanes_) PARODE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     st_.arenaBytes = arena_->capacity();
-    worker_ = std::thread([this] { loop(); });
+    workers_ = lanes < 4 ? lanes : 4;
+    for (int w = 0; w < workers_; ++w) threads_.emplace_back([this, w] { loop(w); });
 }
 
 Server::~Server()
@@ -93,7 +94,8 @@ Server::~Server()
         stop_ = true;
     }
     work_cv_.notify_all();
-    if (worker_.joinable()) worker_.join();
+    for (auto& t : threads_)
+        if (t.joinable()) t.join();
     for (auto& s : lanes_) (void)hipStreamDestroy(s);
 }
 
@@ -187,16 +189,21 @@ bool Server::admit(Job& j, int lane)
 
 void Server::finish(Job& j, int status)
 {
-    if (status == ParODE_OK)
-        std::memcpy(j.r.Y, j.stage + j.L.stY, (size_t)j.r.points * j.L.nOut * 4);
     if (j.done) (void)hipEventDestroy(j.done);
     j.done = nullptr;
     j.core.reset();
     arena_->give(j.io);
-    pinned_->give(j.stage);
-    j.io = j.stage = nullptr;
+    j.io = nullptr;
+    if (status != ParODE_OK) {   // nothing to copy out
+        pinned_->give(j.stage);
+        j.stage = nullptr;
+    }
     std::lock_guard<std::mutex> g(mu_);
-    if (j.lane >= 0) --load_[j.lane];
+    if (j.lane >= 0) {
+        --load_[j.lane];
+        --running_;
+    }
+    if (j.stage) ++staged_;
     j.tDone = Clock::now();
     j.status = status;
     if (status == ParODE_OK)
@@ -206,34 +213,38 @@ void Server::finish(Job& j, int status)
     done_cv_.notify_all();
 }
 
-void Server::loop()
+void Server::loop(int worker)
 {
     if (hipSetDevice(device_) != hipSuccess) return;
+    std::vector<std::shared_ptr<Job>> running;   // this worker's requests on the GPU
     for (;;) {
         bool progressed = false;
         // retire the finished requests
-        for (size_t i = 0; i < running_.size();) {
-            Job& j = *running_[i];
+        for (size_t i = 0; i < running.size();) {
+            Job& j = *running[i];
             const hipError_t e = hipEventQuery(j.done);
             if (e == hipErrorNotReady) {
                 ++i;
                 continue;
             }
             finish(j, e == hipSuccess ? ParODE_OK : ParODE_NotEnoughResources);
-            running_.erase(running_.begin() + i);
+            running.erase(running.begin() + i);
             progressed = true;
         }
-        // admit in arrival order while a lane has room and memory allows
+        // admit in arrival order while one of this worker's lanes has room
         for (;;) {
             std::shared_ptr<Job> j;
             int lane = -1;
             {
                 std::lock_guard<std::mutex> g(mu_);
                 if (queue_.empty()) break;
-                for (int l = 0; l < (int)lanes_.size(); ++l)
+                for (int l = worker; l < (int)lanes_.size(); l += workers_)
                     if (load_[l] < depth_ && (lane < 0 || load_[l] < load_[lane])) lane = l;
                 if (lane < 0) break;
                 j = queue_.front();
+                queue_.pop_front();
+                ++load_[lane];
+                ++active_;
             }
             bool ok = false;
             int err = ParODE_OK;
@@ -244,30 +255,44 @@ void Server::loop()
                 err = ParODE_NotEnoughResources;
                 (void)hipStreamSynchronize(lanes_[lane]);   // nothing of it may still run
             }
-            if (!ok && err == ParODE_OK && !running_.empty()) break;   // wait for memory
+            if (!ok) {
+                bool requeued = false;
+                {
+                    std::lock_guard<std::mutex> g(mu_);
+                    --load_[lane];
+                    --active_;
+                    // no memory now: back to the head of the queue while
+                    // anything else holds memory (running, being admitted, or
+                    // outputs not yet collected); else it can never fit
+                    if (err == ParODE_OK &&
+                        (running_ > 0 || active_ > 0 || staged_ > 0 || !running.empty())) {
+                        queue_.push_front(j);
+                        requeued = true;
+                    }
+                }
+                if (requeued) break;
+                finish(*j, err == ParODE_OK ? ParODE_NotEnoughResources : err);
+                progressed = true;
+                continue;
+            }
             {
                 std::lock_guard<std::mutex> g(mu_);
-                queue_.pop_front();
-                if (ok) {
-                    ++load_[lane];
-                    st_.arenaPeak = std::max(st_.arenaPeak, arena_->in_use());
-                }
+                --active_;
+                ++running_;
+                st_.peakRunning = std::max(st_.peakRunning, running_);
+                st_.arenaPeak = std::max(st_.arenaPeak, arena_->in_use());
             }
-            if (ok) {
-                running_.push_back(j);
-                st_.peakRunning = std::max<long>(st_.peakRunning, (long)running_.size());
-            } else {
-                finish(*j, err == ParODE_OK ? ParODE_NotEnoughResources : err);
-            }
+            running.push_back(j);
             progressed = true;
         }
         std::unique_lock<std::mutex> lk(mu_);
-        st_.running = (long)running_.size();
+        st_.running = running_;
         st_.queued = (long)queue_.size();
-        if (stop_ && queue_.empty() && running_.empty()) break;
+        if (stop_ && queue_.empty() && running.empty()) break;
         if (progressed) continue;
-        if (running_.empty())
-            work_cv_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+        if (running.empty())
+            work_cv_.wait_for(lk, std::chrono::milliseconds(2),
+                              [&] { return stop_ || !queue_.empty(); });
         else
             work_cv_.wait_for(lk, std::chrono::microseconds(20));
     }
@@ -284,6 +309,19 @@ int Server::wait(long ticket, double* latencyMs, double* queueMs)
     if (latencyMs) *latencyMs = ms(j->tDone - j->tSubmit).count();
     if (queueMs) *queueMs = ms(j->tAdmit - j->tSubmit).count();
     jobs_.erase(ticket);
+    lk.unlock();
+    // the outputs leave pinned staging on the caller's thread
+    if (j->stage) {
+        if (j->status == ParODE_OK)
+            std::memcpy(j->r.Y, j->stage + j->L.stY, (size_t)j->r.points * j->L.nOut * 4);
+        pinned_->give(j->stage);
+        j->stage = nullptr;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            --staged_;
+        }
+        work_cv_.notify_all();   // staging space came back
+    }
     return j->status;
 }
 

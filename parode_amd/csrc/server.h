@@ -1,17 +1,19 @@
 // Continuous-batching request loop for LTI solves on one GPU.
 //
 // Requests (one system each: method, shape, horizon, matrices, inputs) are
-// submitted from any thread and run on a worker thread that owns the GPU
-// work.  The worker admits queued requests in arrival order whenever a lane
+// submitted from any thread and run on worker threads that own the GPU work.
+// A worker admits queued requests in arrival order whenever a lane
 // (a HIP stream) has room and the device arena can hold the request's
 // footprint (scan trajectory, FP64 power table, level tables, inputs and
 // outputs).  An admitted request's whole solve is enqueued on its lane without
 // host synchronisation, so requests on different lanes overlap on the device
-// and new arrivals are admitted while earlier ones run.  A finished request's
-// outputs are copied to the caller's buffer and its memory returns to the
-// arena.  Inputs are either host samples on the h/2 grid or the coefficients
-// of the family u_j = a sin(w t + phi) + b cos(w t + phi) + c + d t (+ noise),
-// sampled in FP64 on the device.
+// and new arrivals are admitted while earlier ones run.  Several worker
+// threads share the admission queue, each driving its own lanes, so kernel
+// launches are issued in parallel.  When a request's GPU work ends, its device
+// memory returns to the arena at once; its outputs wait in pinned staging
+// until the caller's wait() copies them out.  Inputs are either host samples
+// on the h/2 grid or the coefficients of the family u_j = a sin(w t + phi) +
+// b cos(w t + phi) + c + d t (+ noise), sampled in FP64 on the device.
 #pragma once
 #include <chrono>
 #include <condition_variable>
@@ -66,23 +68,22 @@ public:
 private:
     struct Job;
     using Clock = std::chrono::steady_clock;
-    void loop();
+    void loop(int worker);
     bool admit(Job& j, int lane);
     void finish(Job& j, int status);
 
-    int device_ = 0, precision_ = 0, depth_ = 2;
+    int device_ = 0, precision_ = 0, depth_ = 2, workers_ = 1;
     std::vector<hipStream_t> lanes_;
     std::vector<int> load_;
     std::unique_ptr<Arena> arena_, pinned_;
     std::mutex mu_;
     std::condition_variable work_cv_, done_cv_;
     std::deque<std::shared_ptr<Job>> queue_;
-    std::vector<std::shared_ptr<Job>> running_;
     std::unordered_map<long, std::shared_ptr<Job>> jobs_;
-    long next_ = 1;
+    long next_ = 1, running_ = 0, active_ = 0, staged_ = 0;
     bool stop_ = false;
     ServeStats st_;
-    std::thread worker_;
+    std::vector<std::thread> threads_;
 };
 
 }  // namespace parode
