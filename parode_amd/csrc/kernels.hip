@@ -663,12 +663,26 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
     const long perBlock = (long)groups * kLeafEB;
     for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
         const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
-        // stage the inputs of the block's elements
-        for (int t = threadIdx.x; t < cnt * km; t += 256) {
-            const int le = t / km, c = t - le * km;
-            const int q = c / m, j = c - q * m;
-            const long e = eb + le;
-            Us[t] = e < nReal ? (TA)U[(e * sL + oL + (long)q * rs) * m + j] : (TA)0;
+        // stage the inputs of the block's elements (eight loads in flight per thread)
+        constexpr int SU = 8;
+        for (int t0 = 0; t0 < cnt * km; t0 += 256 * SU) {
+            TA vals[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int t = t0 + threadIdx.x + u * 256;
+                vals[u] = (TA)0;
+                if (t < cnt * km) {
+                    const int le = t / km, c = t - le * km;
+                    const int q = c / m, j = c - q * m;
+                    const long e = eb + le;
+                    if (e < nReal) vals[u] = (TA)U[(e * sL + oL + (long)q * rs) * m + j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int t = t0 + threadIdx.x + u * 256;
+                if (t < cnt * km) Us[t] = vals[u];
+            }
         }
         // zero rows [0, off) and [off + n, na) of the block's elements; when the
         // first level reads only rows from zlo on, rows [zlo, off) of real
@@ -692,7 +706,22 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
             TA v[kLeafEB];
 #pragma unroll
             for (int u = 0; u < kLeafEB; ++u) v[u] = 0;
-            for (int c = 0; c < km; ++c) {
+            // Bbar columns eight at a time (loads issued together), summed in
+            // column order
+            int c = 0;
+            for (; c + 8 <= km; c += 8) {
+                TA b[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) b[q] = (TA)BbT[(long)(c + q) * n + r];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+#pragma unroll
+                    for (int u = 0; u < kLeafEB; ++u) {
+                        TA pr = b[q] * ug[u * km + c + q];
+                        v[u] = v[u] + pr;
+                    }
+            }
+            for (; c < km; ++c) {
                 const TA b = (TA)BbT[(long)c * n + r];
 #pragma unroll
                 for (int u = 0; u < kLeafEB; ++u) {
