@@ -95,9 +95,8 @@ def batch_legs(args, torch, api, synth):
     plain  : FP32 plain precision, one system per solve, the solves queued back
              to back on one stream, inputs and outputs resident in HBM.
     served : the continuous-batching server, mixed precision, all requests
-             submitted at once (largest first); end to end (host matrices in,
-             host outputs out) from the first submission to the last
-             completion.
+             submitted at once; end to end (host matrices in, host outputs
+             out) from the first submission to the last completion.
     stream : the server under Poisson arrivals from the same distribution;
              p50/p99 latency from submission to outputs on the host."""
     dev = torch.device("cuda:0")
@@ -146,9 +145,7 @@ def batch_legs(args, torch, api, synth):
     srv = api.Server(lanes=args.lanes, device_bytes=int(need * 1.1) + (1 << 30),
                      pinned_bytes=2 << 30, precision=api.MIXED)
 
-    # the whole batch is known up front: submit the largest solves first (the
-    # server admits in arrival order), so the longest ones do not start last
-    order = sorted(range(len(systems)), key=lambda i: -systems[i].steps * systems[i].n_scan ** 2)
+    order = list(range(len(systems)))      # arrival order = generation order
 
     def serve_all():
         t0 = time.perf_counter()
