@@ -630,6 +630,16 @@ void powers_small(hipStream_t s, int na, int L, double* PT)
     check_launch("powers_small");
 }
 
+// a * b + v: separately rounded in FP32 (the reference's float arithmetic,
+// mirrored by the plain path), fused in FP64 (the precision paths).
+__device__ __forceinline__ float madd(float a, float b, float v)
+{
+#pragma clang fp contract(off)
+    const float pr = a * b;
+    return v + pr;
+}
+__device__ __forceinline__ double madd(double a, double b, double v) { return fma(a, b, v); }
+
 // ------------------------------------------------------------------ leaves
 template <typename T>
 struct Acc {
@@ -716,18 +726,12 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
 #pragma unroll
-                    for (int u = 0; u < kLeafEB; ++u) {
-                        TA pr = b[q] * ug[u * km + c + q];
-                        v[u] = v[u] + pr;
-                    }
+                    for (int u = 0; u < kLeafEB; ++u) v[u] = madd(b[q], ug[u * km + c + q], v[u]);
             }
             for (; c < km; ++c) {
                 const TA b = (TA)BbT[(long)c * n + r];
 #pragma unroll
-                for (int u = 0; u < kLeafEB; ++u) {
-                    TA pr = b * ug[u * km + c];
-                    v[u] = v[u] + pr;
-                }
+                for (int u = 0; u < kLeafEB; ++u) v[u] = madd(b, ug[u * km + c], v[u]);
             }
 #pragma unroll
             for (int u = 0; u < kLeafEB; ++u)
@@ -1611,16 +1615,10 @@ __global__ __launch_bounds__(256) void k_outputs(int n, int m, int p, int na, in
             }
             TA v = 0;
             if (useC)
-                for (int c = 0; c < n; ++c) {
-                    TA pr = (TA)cs[c * p + j] * (TA)x[c];
-                    v = v + pr;
-                }
+                for (int c = 0; c < n; ++c) v = madd((TA)cs[c * p + j], (TA)x[c], v);
             if (useD) {
                 const TU* u = us + le * m;
-                for (int c = 0; c < m; ++c) {
-                    TA pr = (TA)ds[c * p + j] * (TA)u[c];
-                    v = v + pr;
-                }
+                for (int c = 0; c < m; ++c) v = madd((TA)ds[c * p + j], (TA)u[c], v);
             }
             y[j] = (float)v;
         }
