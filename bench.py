@@ -24,24 +24,25 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 
-# HBM bytes of one level-0 up-sweep launch of the level kernel at the default
-# workload (2^19 pairs of 256-float columns), from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters, FETCH scaled by 2.0 (what the counter reads for a plain
-# copy of known size): (529458 * 2.0 + 524287) KiB.
-TRAFFIC_LEVEL0_UP = (529458 * 2.0 + 524287) * 1024
+# HBM bytes of the largest launch of the level kernel at the default workload,
+# level 0 of the down-sweep (2^19 pairs of 256-float columns), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE counters, FETCH scaled by 2.0 (what the counter
+# reads for a plain copy of known size): (577050 * 2.0 + 1048574) KiB.
+TRAFFIC_LEVEL0_DOWN = (577050 * 2.0 + 1048574) * 1024
 
 
 def level_flops(na, n_scan, n_leaf=None):
     """Algorithmic flops of the level sweeps: 2 na^2 per pair, pairs of
     level l = ceil(N / 2^(l+1)), both sweeps.  With n_leaf (ABM: the n
-    nonzero rows of each n_a-row leaf), level 0 of the up-sweep costs
-    2 na n_leaf per pair: only those source states are nonzero."""
+    nonzero rows of each n_a-row leaf), level 0 costs 2 na n_leaf per pair in
+    each sweep: in the up-sweep only those source states are nonzero, in the
+    down-sweep only those rows of each state are read by the outputs."""
     L = max(0, (n_scan - 1).bit_length())
     per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
     pairs = sum(per)
     flops = 2 * 2.0 * na * na * pairs
     if n_leaf and L:
-        flops -= 2.0 * na * (na - n_leaf) * per[0]
+        flops -= 2 * 2.0 * na * (na - n_leaf) * per[0]
     return flops, pairs
 
 
@@ -347,11 +348,15 @@ def main():
                    "precision": args.precision, "points": s.steps, "scan_size": na},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": TRAFFIC_LEVEL0_UP if (args.config == 1 and not args.points)
+                     "traffic": TRAFFIC_LEVEL0_DOWN if (args.config == 1 and not args.points)
                      else None,
+                     "traffic_launch": "level 0 of the down-sweep",
                      "kernel": "k_level_mfma (every level launch of both sweeps)",
                      "flops_per_solve": flops, "pairs_per_sweep": pairs, "sweep_ms": sweep_ms,
-                     "algorithmic_bytes_level0_up": 3 * (n_scan // 2) * na * 4},
+                     # read X[R] and the leaf rows of X[L] that are not zero, write
+                     # the rows of both that the outputs read
+                     "algorithmic_bytes_level0_down":
+                         (n_scan // 2) * 4 * (na + 3 * s.n if s.method == 1 else 4 * na)},
         "phases_ms": ph,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plan_bytes": plan.footprint,

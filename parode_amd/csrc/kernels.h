@@ -59,6 +59,20 @@ template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int k, int na, int off, long nReal, long nPad,
             const TB* Bbar, const TU* U, long sL, long oL, long rs, T* X, bool writeZeros = true);
 
+// Leaves whose Bbar factors as Bf (G (x) I_m): Bbar_q = sum_g G[q][g] Bf_g for
+// a kq x kg matrix G (kq <= 5, kg <= 2).  ABM (FP64 map): Bf = [B | AB] and
+// G from abm_leaf_factors, so each leaf is 2m products per row instead of 5m,
+// in FP64 on the matrix cores.  BfT is Bf transposed, [kg m][n].  Returns
+// false (nothing launched) when the shape does not fit the kernel.
+struct LeafComb {
+    double g[5][2];
+};
+void abm_leaf_factors(double h, LeafComb& G);
+template <typename T, typename TU>
+bool leaves_factored(hipStream_t s, int n, int m, int kq, int kg, int na, int off, long nReal,
+                     long nPad, const double* BfT, const LeafComb& G, const TU* U, long sL,
+                     long oL, long rs, T* X, bool writeZeros = true);
+
 // ------------------------------------------------------------ scan levels
 // Up-sweep level with stride d over np pairs: X[R] += P X[L] (R = (k+1)2d-1,
 // L = R-d); with delta tables X[R] += X[L] + (P-I) X[L].
@@ -75,10 +89,13 @@ void make_level_table_padded(hipStream_t s, int na, int nap, int nrp, const doub
 // rbeg: first row that is not zero in the level-0 leaves (ABM: 3n; the rows
 // below are not read); jbeg: first source state the up-sweep product needs (a
 // multiple of 32, <= rbeg; 0 for the down-sweep).  Both 0 in general.
-// For na > 256 (level_mfma_splits) the rows are split over work-groups, and
-// the down-sweep first copies the pairs' X[R] into S (np * na floats).
+// ibase: the down-sweep writes rows i >= ibase only (a multiple of 32, <= rbeg;
+// ABM level 0: the rows the outputs read); 0 in general and in the up-sweep.
+// For more than 256 rows (na > 256: level_mfma_splits) the rows are split over
+// work-groups, and the down-sweep first copies the pairs' X[R] into S
+// (np * na floats).
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, int jbeg = 0, int rbeg = 0, float* S = nullptr);
+                float* X, int jbeg = 0, int rbeg = 0, float* S = nullptr, int ibase = 0);
 inline bool level_mfma_splits(int na) { return na > 256; }
 
 // A level with few pairs as many small K-split tiles (32 x 32 FP32 MFMA when

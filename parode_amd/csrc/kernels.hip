@@ -646,17 +646,20 @@ struct Acc {
     using type = T;
 };
 
-// A work-group takes EB consecutive elements per thread group: the inputs of
-// its elements are staged in LDS, and each thread forms one row r (< n) of all
-// EB leaves, so every Bbar value it loads is used EB times.  Thread groups are
-// 2^tpeLog >= n wide (<= 256); index math is shifts.  Rows of X outside
+// A work-group takes EB consecutive elements per thread group.  The sample
+// rows its elements read form one contiguous run of U (element e reads rows
+// e*sL + oL + q*rs, q < kq), staged once in LDS; each thread forms one row r
+// (< n) of all EB leaves, so every Bbar value it loads is used EB times.
+// Thread groups are 2^tpeLog >= n wide (<= 256); index math is shifts and
+// products (no divisions).  Leaves past nReal are zero.  Rows of X outside
 // [off, off + n) (the zero blocks of the ABM companion state) are written by
-// all threads of the work-group together.
+// each thread group for its own elements.
 constexpr int kLeafEB = 8;
 
 template <typename T, typename TU, typename TB>
 __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
-                                                long nPad, int tpeLog, bool writeZeros, int zlo,
+                                                long nPad, int tpeLog, int groups,
+                                                bool writeZeros, int zlo,
                                                 const TB* __restrict__ BbT,
                                                 const TU* __restrict__ U, long sL, long oL,
                                                 long rs, T* __restrict__ X)
@@ -664,82 +667,315 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
 #pragma clang fp contract(off)
     using TA = decltype(TB(0) * TU(0));
     extern __shared__ unsigned char leaf_smem[];
-    TA* Us = reinterpret_cast<TA*>(leaf_smem);       // [groups*EB][kq*m]
+    TA* Us = reinterpret_cast<TA*>(leaf_smem);       // staged rows x m
     const int tpe = 1 << tpeLog;
-    const int groups = 256 >> tpeLog;
     const int grp = threadIdx.x >> tpeLog, lane = threadIdx.x & (tpe - 1);
-    const int km = kq * m;
-    const int nz = na - n;                            // zero rows per element
     const long perBlock = (long)groups * kLeafEB;
+    // the last sample row a real leaf reads
+    const long lastRow = (nReal - 1) * sL + oL + (long)(kq - 1) * rs;
+    const long es = sL * m;                           // staged values between elements
     for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
         const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
-        // stage the inputs of the block's elements (eight loads in flight per thread)
+        const long row0 = eb * sL + oL;
+        long rows = (long)(cnt - 1) * sL + (long)(kq - 1) * rs + 1;
+        if (rows > lastRow - row0 + 1) rows = lastRow - row0 + 1;
+        const int vals = rows > 0 ? (int)(rows * m) : 0;
+        const TU* src = U + row0 * m;
+        // stage the block's sample rows (eight loads in flight per thread)
         constexpr int SU = 8;
-        for (int t0 = 0; t0 < cnt * km; t0 += 256 * SU) {
-            TA vals[SU];
+        for (int t0 = 0; t0 < vals; t0 += 256 * SU) {
+            TA v[SU];
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
                 const int t = t0 + threadIdx.x + u * 256;
-                vals[u] = (TA)0;
-                if (t < cnt * km) {
-                    const int le = t / km, c = t - le * km;
-                    const int q = c / m, j = c - q * m;
-                    const long e = eb + le;
-                    if (e < nReal) vals[u] = (TA)U[(e * sL + oL + (long)q * rs) * m + j];
-                }
+                v[u] = t < vals ? (TA)src[t] : (TA)0;
             }
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
                 const int t = t0 + threadIdx.x + u * 256;
-                if (t < cnt * km) Us[t] = vals[u];
+                if (t < vals) Us[t] = v[u];
             }
         }
-        // zero rows [0, off) and [off + n, na) of the block's elements; when the
-        // first level reads only rows from zlo on, rows [zlo, off) of real
-        // elements (and every zero row of padding elements)
-        const long firstZ = writeZeros ? eb : (nReal > eb ? nReal : eb);
-        const int z0 = firstZ - eb < cnt ? (int)(firstZ - eb) * nz : cnt * nz;
-        for (int t = z0 + threadIdx.x; t < cnt * nz; t += 256) {
-            const int le = t / nz, z = t - le * nz;
-            X[(eb + le) * na + (z < off ? z : z + n)] = (T)0;
-        }
-        const int nzl = off - zlo;                 // rows [zlo, off) of real elements
-        const int creal = writeZeros ? 0 : (int)((firstZ - eb) < cnt ? firstZ - eb : cnt);
-        for (int t = threadIdx.x; t < creal * nzl; t += 256) {
-            const int le = t / nzl, z = t - le * nzl;
-            X[(eb + le) * na + zlo + z] = (T)0;
+        const int le0 = grp * kLeafEB;
+        const long e0 = eb + le0;
+        if (grp < groups && le0 < cnt) {
+            // zero rows [0, off) and [off + n, na) of the group's elements; when
+            // the first level reads only rows from zlo on, rows [zlo, off) of
+            // real elements and every zero row of padding elements
+            const int ne = (cnt - le0) < kLeafEB ? cnt - le0 : kLeafEB;
+            for (int u = 0; u < ne; ++u) {
+                T* xe = X + (e0 + u) * na;
+                const bool all = writeZeros || e0 + u >= nReal;
+                for (int z = (all ? 0 : zlo) + lane; z < off; z += tpe) xe[z] = (T)0;
+                if (all)
+                    for (int z = off + n + lane; z < na; z += tpe) xe[z] = (T)0;
+            }
         }
         __syncthreads();
-        const long e0 = eb + (long)grp * kLeafEB;
-        const TA* ug = Us + (long)grp * kLeafEB * km;
-        for (int r = lane; r < n && e0 < nPad; r += tpe) {
-            TA v[kLeafEB];
+        if (grp < groups && le0 < cnt) {
+            for (int r = lane; r < n; r += tpe) {
+                TA v[kLeafEB];
 #pragma unroll
-            for (int u = 0; u < kLeafEB; ++u) v[u] = 0;
-            // Bbar columns eight at a time (loads issued together), summed in
-            // column order
-            int c = 0;
-            for (; c + 8 <= km; c += 8) {
-                TA b[8];
+                for (int u = 0; u < kLeafEB; ++u) v[u] = 0;
+                // Bbar columns c = q m + j in order, eight at a time (loads
+                // issued together)
+                for (int q = 0; q < kq; ++q) {
+                    const TA* ub = Us + ((long)le0 * sL + (long)q * rs) * m;
+                    const TB* bq = BbT + (long)q * m * n + r;
+                    int j = 0;
+                    for (; j + 8 <= m; j += 8) {
+                        TA b[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) b[q] = (TA)BbT[(long)(c + q) * n + r];
+                        for (int jj = 0; jj < 8; ++jj) b[jj] = (TA)bq[(long)(j + jj) * n];
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
+                        for (int jj = 0; jj < 8; ++jj)
 #pragma unroll
-                    for (int u = 0; u < kLeafEB; ++u) v[u] = madd(b[q], ug[u * km + c + q], v[u]);
+                            for (int u = 0; u < kLeafEB; ++u)
+                                v[u] = madd(b[jj], ub[u * es + j + jj], v[u]);
+                    }
+                    for (; j < m; ++j) {
+                        const TA b = (TA)bq[(long)j * n];
+#pragma unroll
+                        for (int u = 0; u < kLeafEB; ++u) v[u] = madd(b, ub[u * es + j], v[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kLeafEB; ++u)
+                    if (le0 + u < cnt)
+                        X[(e0 + u) * na + off + r] = e0 + u < nReal ? (T)v[u] : (T)0;
             }
-            for (; c < km; ++c) {
-                const TA b = (TA)BbT[(long)c * n + r];
-#pragma unroll
-                for (int u = 0; u < kLeafEB; ++u) v[u] = madd(b, ug[u * km + c], v[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < kLeafEB; ++u)
-                if (e0 + u < nPad) X[(e0 + u) * na + off + r] = (T)v[u];
         }
         __syncthreads();
     }
 }
+
+// FP64 leaves on the matrix cores (v_mfma_f64_16x16x4f64), as the transposed
+// product V^T[e][r] = sum_c W[e][c] Bf[r][c] per tile of 16 leaves x 16 rows:
+//   A (16 leaves x 4 columns): lane l holds W[e0 + l%16][c0 + l/16]
+//   B (4 columns x 16 rows):   lane l holds Bf[r0 + l%16][c0 + l/16]
+//   D: lane l, register v holds leaf e0 + 4v + l/16, row r0 + l%16
+// The sample rows of the work-group's leaves (one contiguous run of U) are
+// staged in LDS as FP64 with an odd row pitch (the 16 leaves of a fragment
+// fall in distinct banks).  Without FAC, W is the leaf's samples themselves:
+// column c = q m + j is input j of sample row e*sL + oL + q*rs (the lane's
+// (q, j) advance by four columns per k-step without divisions) and Bf = Bbar.
+// With FAC, Bbar = Bf (G (x) I_m) for a kq x kg matrix G (ABM: Bf = [B | AB]),
+// and W[e][g m + j] = sum_q G[q][g] u_{e,q}[j] is formed in LDS first: kg m
+// columns instead of kq m.  A wave takes one row tile of kLeafTiles leaf
+// tiles; the work-group's waves cover the row tiles of G groups of leaf tiles.
+// B fragments are loaded eight k-steps at a time.  Leaves past nReal are
+// stored as zero.  Rows outside [off, off + n) are zero as in k_leaves.
+constexpr int kLeafTiles = 4;
+constexpr int kLeafMaxQ = 5, kLeafMaxG = 2;
+
+template <typename T, typename TU, bool FAC>
+__global__ __launch_bounds__(256) void k_leaves_mfma(int n, int m, int kq, int na, int off,
+                                                     long nReal, long nPad, int groups,
+                                                     int pitch, unsigned magic, int kg,
+                                                     LeafComb G, bool writeZeros, int zlo,
+                                                     const double* __restrict__ BbT,
+                                                     const TU* __restrict__ U, long sL, long oL,
+                                                     long rs, T* __restrict__ X)
+{
+    extern __shared__ unsigned char leaf_smem[];
+    double* Us = reinterpret_cast<double*>(leaf_smem);   // staged rows x pitch
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, k4 = lane >> 4;
+    const int km = FAC ? kg * m : kq * m, ks = (km + 3) / 4;
+    const int rowTiles = (n + 15) / 16;
+    const long perGroup = 16L * kLeafTiles, perBlock = perGroup * groups;
+    const long lastRow = (nReal - 1) * sL + oL + (long)(kq - 1) * rs;
+    const long stagedMax = (perBlock - 1) * sL + (long)(kq - 1) * rs + 1;
+    const int pw = km | 1;                                  // W pitch (FAC)
+    double* Ws = Us + stagedMax * pitch;                    // perBlock x pw (FAC)
+    const int nz = na - n;
+    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
+        const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
+        const long row0 = eb * sL + oL;
+        long rows = (long)(cnt - 1) * sL + (long)(kq - 1) * rs + 1;
+        if (rows > lastRow - row0 + 1) rows = lastRow - row0 + 1;
+        const int vals = rows > 0 ? (int)(rows * m) : 0;
+        const TU* src = U + row0 * m;
+        // stage (eight loads in flight per thread); row t / m by multiply-high
+        constexpr int SU = 8;
+        for (int t0 = 0; t0 < vals; t0 += 256 * SU) {
+            double v[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int t = t0 + threadIdx.x + u * 256;
+                v[u] = t < vals ? (double)src[t] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int t = t0 + threadIdx.x + u * 256;
+                const int r = m == 1 ? t : (int)__umulhi((unsigned)t, magic);
+                if (t < vals) Us[r * pitch + (t - r * m)] = v[u];
+            }
+        }
+        // zero rows [0, off) and [off + n, na); when the first level reads only
+        // rows from zlo on, rows [zlo, off) of real elements and every zero row
+        // of padding elements
+        if (nz > 0) {
+            for (int le = 0; le < cnt; ++le) {
+                T* xe = X + (eb + le) * na;
+                const bool all = writeZeros || eb + le >= nReal;
+                const int zb = all ? 0 : zlo, ze = all ? nz : off;
+                for (int z = zb + (int)threadIdx.x; z < ze; z += 256)
+                    xe[z < off ? z : z + n] = (T)0;
+            }
+        }
+        __syncthreads();
+        // the real leaves of the block (the others are stored as zero)
+        const int creal = nReal - eb < cnt ? (int)(nReal - eb > 0 ? nReal - eb : 0) : cnt;
+        if (FAC) {
+            // W[le][g m + j] = sum_q G[q][g] u[le*sL + q*rs][j], q in order
+            for (int t = threadIdx.x; t < creal * m; t += 256) {
+                const int le = m == 1 ? t : (int)__umulhi((unsigned)t, magic), j = t - le * m;
+                const double* ur = Us + (long)le * sL * pitch + j;
+                for (int g = 0; g < kg; ++g) {
+                    double w = 0.0;
+                    for (int q = 0; q < kq; ++q) w = fma(G.g[q][g], ur[(long)q * rs * pitch], w);
+                    Ws[le * pw + g * m + j] = w;
+                }
+            }
+            __syncthreads();
+        }
+        for (int item = wave; item < rowTiles * groups; item += 4) {
+            const int rt = item % rowTiles, g = item / rowTiles;
+            const int le0 = g * (int)perGroup;
+            if (le0 >= cnt) continue;
+            const int r0 = rt * 16;
+            const int rr = (r0 + l16) < n ? r0 + l16 : n - 1;
+            // LDS offset of this lane's leaf in each tile, clamped to a real
+            // leaf of the block (products of the others are not stored)
+            int ub[kLeafTiles];
+#pragma unroll
+            for (int t = 0; t < kLeafTiles; ++t) {
+                int le = le0 + t * 16 + l16;
+                le = le < creal - 1 ? le : (creal > 0 ? creal - 1 : 0);
+                ub[t] = FAC ? le * pw : (int)(le * sL) * pitch;
+            }
+            doublex4 acc[kLeafTiles];
+#pragma unroll
+            for (int t = 0; t < kLeafTiles; ++t) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+            int q = 0, j = k4;
+            if (!FAC)
+                while (j >= m) { j -= m; ++q; }
+            for (int kb = 0; kb < ks; kb += 8) {
+                double b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int c = 4 * (kb + u) + k4;
+                    b[u] = c < km ? BbT[(long)c * n + rr] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (kb + u >= ks) break;
+                    const int c = 4 * (kb + u) + k4;
+                    const bool in = c < km;
+                    const int cu = FAC ? c : (int)(q * rs) * pitch + j;
+                    double a[kLeafTiles];
+#pragma unroll
+                    for (int t = 0; t < kLeafTiles; ++t) a[t] = in ? (FAC ? Ws : Us)[ub[t] + cu] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < kLeafTiles; ++t)
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t], 0, 0, 0);
+                    if (!FAC) {
+                        j += 4;
+                        while (j >= m) { j -= m; ++q; }
+                    }
+                }
+            }
+            if (r0 + l16 >= n) continue;
+#pragma unroll
+            for (int t = 0; t < kLeafTiles; ++t)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int le = le0 + t * 16 + 4 * v + k4;
+                    if (le < cnt)
+                        X[(eb + le) * na + off + r0 + l16] = le < creal ? (T)acc[t][v] : (T)0;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// LDS bytes of one work-group of k_leaves_mfma: the staged sample rows, and
+// with kg > 0 the combined inputs W
+static size_t leaves_mfma_lds(long perBlock, int m, int kq, int kg, long sL, long rs, int pitch)
+{
+    size_t d = (size_t)((perBlock - 1) * sL + (long)(kq - 1) * rs + 1) * pitch;
+    if (kg > 0) d += (size_t)perBlock * ((kg * m) | 1);
+    return d * sizeof(double);
+}
+
+template <typename T, typename TU>
+static bool leaves_mfma(hipStream_t s, int n, int m, int kq, int na, int off, long nReal,
+                        long nPad, const double* Bf, int kg, const LeafComb& G, const TU* U,
+                        long sL, long oL, long rs, T* X, bool writeZeros)
+{
+    const int rowTiles = (n + 15) / 16;
+    int groups = rowTiles >= 4 ? 1 : 4 / rowTiles;
+    const int pitch = m | 1;
+    while (groups > 1 &&
+           leaves_mfma_lds(16L * kLeafTiles * groups, m, kq, kg, sL, rs, pitch) > 65536)
+        groups >>= 1;
+    const long perBlock = 16L * kLeafTiles * groups;
+    const size_t lds = leaves_mfma_lds(perBlock, m, kq, kg, sL, rs, pitch);
+    // the staged row index t / m (and the leaf index of W) is exact by
+    // multiply-high for t, m < 2^16 (m = 1: the magic number 2^32 does not fit
+    // 32 bits, and the kernel uses t itself)
+    if (lds > 65536 || m >= 65536 || lds / sizeof(double) >= 65536 || perBlock * m >= 65536)
+        return false;
+    const unsigned magic = (unsigned)(((1ULL << 32) + m - 1) / m);
+    const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
+    const dim3 grid((unsigned)std::max<long>(blocks, 1));
+    if (kg > 0)
+        hipLaunchKernelGGL((k_leaves_mfma<T, TU, true>), grid, dim3(256), lds, s, n, m, kq, na,
+                           off, nReal, nPad, groups, pitch, magic, kg, G, writeZeros,
+                           off / 32 * 32, Bf, U, sL, oL, rs, X);
+    else
+        hipLaunchKernelGGL((k_leaves_mfma<T, TU, false>), grid, dim3(256), lds, s, n, m, kq, na,
+                           off, nReal, nPad, groups, pitch, magic, 0, G, writeZeros,
+                           off / 32 * 32, Bf, U, sL, oL, rs, X);
+    check_launch("leaves_mfma");
+    return true;
+}
+
+void abm_leaf_factors(double h, LeafComb& G)
+{
+    // Bbar_q = G[q][0] B + G[q][1] AB, q = 0..4, as k_abm_combine forms it
+    const AbmCoef c = abm_coef(h);
+    const double g[5][2] = {{0.0, c.b3AB}, {c.b2B, c.b2AB}, {c.b1B, c.b1AB}, {c.b0B, c.b0AB},
+                            {c.bm1, 0.0}};
+    for (int q = 0; q < 5; ++q)
+        for (int k = 0; k < 2; ++k) G.g[q][k] = g[q][k];
+}
+
+template <typename T, typename TU>
+bool leaves_factored(hipStream_t s, int n, int m, int kq, int kg, int na, int off, long nReal,
+                     long nPad, const double* BfT, const LeafComb& G, const TU* U, long sL,
+                     long oL, long rs, T* X, bool writeZeros)
+{
+    if (m <= 0 || kq > kLeafMaxQ || kg < 1 || kg > kLeafMaxG || nReal < 1 || sL < 0 || oL < 0 ||
+        rs < 0)
+        return false;
+    return leaves_mfma<T, TU>(s, n, m, kq, na, off, nReal, nPad, BfT, kg, G, U, sL, oL, rs, X,
+                              writeZeros);
+}
+
+template bool leaves_factored<float, double>(hipStream_t, int, int, int, int, int, int, long,
+                                             long, const double*, const LeafComb&,
+                                             const double*, long, long, long, float*, bool);
+template bool leaves_factored<float, float>(hipStream_t, int, int, int, int, int, int, long, long,
+                                            const double*, const LeafComb&, const float*, long,
+                                            long, long, float*, bool);
+template bool leaves_factored<double, double>(hipStream_t, int, int, int, int, int, int, long,
+                                              long, const double*, const LeafComb&,
+                                              const double*, long, long, long, double*, bool);
+template bool leaves_factored<double, float>(hipStream_t, int, int, int, int, int, int, long,
+                                             long, const double*, const LeafComb&, const float*,
+                                             long, long, long, double*, bool);
 
 template <typename T, typename TU, typename TB>
 void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, long nPad,
@@ -752,14 +988,28 @@ void leaves(hipStream_t s, int n, int m, int kq, int na, int off, long nReal, lo
         check_launch("leaves(zero)");
         return;
     }
+    if (nReal < 1 || sL < 0 || oL < 0 || rs < 0)
+        throw std::invalid_argument("leaves: bad sample layout");
+    if constexpr (std::is_same<TB, double>::value) {   // FP64 products on the matrix cores
+        if (leaves_mfma<T, TU>(s, n, m, kq, na, off, nReal, nPad, Bbar, 0, LeafComb{}, U, sL, oL,
+                               rs, X, writeZeros))
+            return;
+    }
     int tpeLog = 0;
     while ((1 << tpeLog) < n && tpeLog < 8) ++tpeLog;
-    const long perBlock = (256L >> tpeLog) * kLeafEB;
+    // thread groups in use: as many as 256 threads hold, fewer when the staged
+    // rows of their elements would pass 64 KiB of LDS
+    int groups = 256 >> tpeLog;
+    auto lds_for = [&](int g) {
+        return (size_t)(((long)g * kLeafEB - 1) * sL + (long)(kq - 1) * rs + 1) * m * sizeof(TA);
+    };
+    while (groups > 1 && lds_for(groups) > 65536) groups >>= 1;
+    if (lds_for(groups) > 65536) throw std::invalid_argument("leaves: too many inputs");
+    const long perBlock = (long)groups * kLeafEB;
     const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
-    const size_t lds = (size_t)perBlock * kq * m * sizeof(TA);
     hipLaunchKernelGGL((k_leaves<T, TU, TB>), dim3((unsigned)std::max<long>(blocks, 1)), dim3(256),
-                       lds, s, n, m, kq, na, off, nReal, nPad, tpeLog, writeZeros, off / 32 * 32,
-                       Bbar, U, sL, oL, rs, X);
+                       lds_for(groups), s, n, m, kq, na, off, nReal, nPad, tpeLog, groups,
+                       writeZeros, off / 32 * 32, Bbar, U, sL, oL, rs, X);
     check_launch("leaves");
 }
 
@@ -922,14 +1172,18 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // 32 floor(3n / 32) the first source state the up-sweep product needs.
 // Elsewhere both are 0.
 //
-// For n_a above one work-group's rows (IB = WGI*NIT*32), blockIdx.y picks a
+// The down-sweep of that level may also produce only rows i >= ibase (a
+// multiple of 32; ABM: the rows the outputs read, the last n): the rows below
+// keep stale values, which nothing reads afterwards.  Elsewhere ibase is 0.
+//
+// For rows beyond one work-group's (IB = WGI*NIT*32), blockIdx.y picks a
 // block of IB rows i.  The up-sweep stays race-free (it reads X[L], writes
 // X[R]).  The down-sweep reads its source columns from S, a compact copy of
 // the old X[R] ([np][na], made by k_gather_r), because other row blocks write
 // X[R] while this one still reads it.  S is null when one block covers n_a.
 template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, long d, long np,
-                                                    int jbeg, int rbeg,
+                                                    int jbeg, int rbeg, int ibase,
                                                     const float* __restrict__ PT,
                                                     const float* __restrict__ S,
                                                     float* __restrict__ X)
@@ -944,7 +1198,7 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     const int wi = wave % WGI, wc = wave / WGI;
     const int l32 = lane & 31, hi = lane >> 5;
     const long k0 = (long)blockIdx.x * TC;
-    const int ib0 = SPLIT ? blockIdx.y * IB : 0;
+    const int ib0 = ibase + (SPLIT ? (int)blockIdx.y * IB : 0);
     constexpr bool fromS = DOWN && SPLIT;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
     const bool full = ncol == TC;
@@ -1107,8 +1361,8 @@ __global__ void k_gather_r(int na, long d, long np, const float* __restrict__ X,
 
 template <int NIT, int NCT, int WGI, bool SPLIT>
 static void mfma_launch_t(hipStream_t s, bool down, bool delta, int na, int nap, int nrp, long d,
-                          long np, int jbeg, int rbeg, const float* PT, float* X, const float* S,
-                          dim3 g)
+                          long np, int jbeg, int rbeg, int ibase, const float* PT, float* X,
+                          const float* S, dim3 g)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
     constexpr int KC = 8192 / TC;
@@ -1116,33 +1370,33 @@ static void mfma_launch_t(hipStream_t s, bool down, bool delta, int na, int nap,
     if (down) {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s, na,
-                               nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+                               nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
     } else {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, PT, S, X);
+                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
     }
     check_launch("level_mfma");
 }
 
 template <int NIT, int NCT, int WGI>
 static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
-                        int jbeg, int rbeg, const float* PT, float* X, float* S)
+                        int jbeg, int rbeg, int ibase, const float* PT, float* X, float* S)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
     constexpr int IB = WGI * NIT * 32;
     const int nap = (na + 31) / 32 * 32, nrp = mfma_table_rows(na);
-    const unsigned gy = (unsigned)((na + IB - 1) / IB);
+    const unsigned gy = (unsigned)((na - ibase + IB - 1) / IB);
     dim3 g((unsigned)((np + TC - 1) / TC), gy);
     if (gy == 1) {
-        mfma_launch_t<NIT, NCT, WGI, false>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, PT, X,
-                                            nullptr, g);
+        mfma_launch_t<NIT, NCT, WGI, false>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, ibase,
+                                            PT, X, nullptr, g);
         return;
     }
     if (down) {
@@ -1151,8 +1405,8 @@ static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, lo
                            d, np, X, S);
         check_launch("gather_r");
     }
-    mfma_launch_t<NIT, NCT, WGI, true>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, PT, X, S,
-                                       g);
+    mfma_launch_t<NIT, NCT, WGI, true>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, ibase, PT,
+                                       X, S, g);
 }
 
 // Rows of the padded MFMA table: a multiple of every K-slice (<= 256).
@@ -1163,29 +1417,30 @@ bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
 // Tile choice by scan size; the narrow variant (32 pairs per work-group)
 // takes over when the wide one would leave the 256 CUs short of work.
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, int jbeg, int rbeg, float* S)
+                float* X, int jbeg, int rbeg, float* S, int ibase)
 {
     if (np <= 0) return;
     if (jbeg % 32 != 0 || jbeg < 0 || jbeg >= na || rbeg < 0 || rbeg >= na || jbeg > rbeg ||
-        (down && jbeg != 0) || (delta && rbeg != 0))
+        (down && jbeg != 0) || (delta && rbeg != 0) || ibase % 32 != 0 || ibase < 0 ||
+        ibase >= na || ibase > rbeg || (!down && ibase != 0))
         throw std::invalid_argument("level_mfma: bad structured range");
-    const int nt = (na + 31) / 32;
+    const int nt = (na - ibase + 31) / 32;
     const bool narrow = np < 2L * 256 * 64;
     if (nt <= 1) {
-        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
-        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
     } else if (nt <= 2) {
-        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
-        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
     } else if (nt <= 4) {
-        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
-        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
     } else if (nt <= 8) {
-        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
-        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, nullptr);
+        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
     } else {
         // above 256 states: 256-row blocks side by side (blockIdx.y)
-        mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, PT, X, S);
+        mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, S);
     }
 }
 
@@ -1717,28 +1972,40 @@ template void outputs<double, float>(hipStream_t, int, int, int, int, int, bool,
 // Three classic RK4 steps (AdamsMoulton.cpp:96-103).  mirror: float state and
 // float derivative sums (AdamsMoulton.cpp:15-27), stage combinations formed
 // in double and stored to float (odeint runge_kutta4<vector<float>>).
+// When A fits (pitch n + 1, no bank conflicts), it is staged in LDS once: the
+// twelve derivative evaluations are chains of n dependent products per row,
+// which from global memory wait on a load each.
 template <typename TU>
-__global__ void k_abm_startup(bool mirror, int n, int m, double h, const float* __restrict__ A,
-                              const float* __restrict__ B, const float* __restrict__ x0,
-                              const TU* __restrict__ Uh, double* __restrict__ X0)
+__global__ void k_abm_startup(bool mirror, int n, int m, double h, bool ldsA,
+                              const float* __restrict__ A, const float* __restrict__ B,
+                              const float* __restrict__ x0, const TU* __restrict__ Uh,
+                              double* __restrict__ X0)
 {
 #pragma clang fp contract(off)
     extern __shared__ double sh[];
     double* x = sh;            // n
     double* xt = sh + n;       // n
     double* kb = sh + 2 * n;   // 4n: k1..k4
+    float* As = reinterpret_cast<float*>(sh + 6 * n);   // n x (n + 1) when ldsA
     const double dh = h / 2.0, dt6 = h / 6.0, dt3 = h / 3.0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
         x[r] = (double)x0[r];
         X0[r] = (double)x0[r];
     }
+    if (ldsA)
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            const int r = t / n;
+            As[t + r] = A[t];   // row r at r * (n + 1)
+        }
     __syncthreads();
+    const float* Ar = ldsA ? As : A;
+    const long ap = ldsA ? n + 1 : n;
     auto deriv = [&](const double* xs, int urow, double* out) {
         for (int r = threadIdx.x; r < n; r += blockDim.x) {
             if (mirror) {
                 float f = 0.0f;
                 for (int c = 0; c < n; ++c) {
-                    float pr = A[(long)r * n + c] * (float)xs[c];
+                    float pr = Ar[r * ap + c] * (float)xs[c];
                     f = f + pr;
                 }
                 for (int c = 0; c < m; ++c) {
@@ -1748,7 +2015,7 @@ __global__ void k_abm_startup(bool mirror, int n, int m, double h, const float* 
                 out[r] = (double)f;
             } else {
                 double f = 0.0;
-                for (int c = 0; c < n; ++c) f = fma((double)A[(long)r * n + c], xs[c], f);
+                for (int c = 0; c < n; ++c) f = fma((double)Ar[r * ap + c], xs[c], f);
                 for (int c = 0; c < m; ++c)
                     f = fma((double)B[(long)r * m + c], (double)Uh[(long)urow * m + c], f);
                 out[r] = f;
@@ -1794,8 +2061,11 @@ void abm_startup(hipStream_t s, bool mirror, int n, int m, double h, const float
                  const float* B, const float* x0, const TU* Uh, double* X0)
 {
     size_t lds = (size_t)6 * n * sizeof(double);
-    hipLaunchKernelGGL((k_abm_startup<TU>), dim3(1), dim3(256), lds, s, mirror, n, m, h, A, B, x0,
-                       Uh, X0);
+    const size_t ldsA = (size_t)n * (n + 1) * sizeof(float);
+    const bool useA = lds + ldsA <= 65536;
+    if (useA) lds += ldsA;
+    hipLaunchKernelGGL((k_abm_startup<TU>), dim3(1), dim3(256), lds, s, mirror, n, m, h, useA, A,
+                       B, x0, Uh, X0);
     check_launch("abm_startup");
 }
 

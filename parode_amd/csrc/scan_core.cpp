@@ -27,6 +27,7 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     Bbar32_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
     BbT64_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
     BbT32_.alloc((size_t)n * kq * (m > 0 ? m : 1), arena);
+    if (method == kABM && opt_.precision != kPlain32 && m > 0) BfT64_.alloc(2 * nm, arena);
     CT_.alloc((size_t)n * (p > 0 ? p : 1), arena);
     DT_.alloc((size_t)(m > 0 ? m : 1) * (p > 0 ? p : 1), arena);
     PT64_.alloc(na2 * (L_ + 1), arena);
@@ -67,6 +68,7 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     size_t b = (opt.precision == kPlain32 ? 4 : 8) * (4 * nn + 4 * nm);
     if (opt.precision != kPlain32) b += 8 * (nn + nm);
     b += 24 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
+    if (method == kABM && opt.precision != kPlain32 && m > 0) b += 16 * nm;
     b += 8 * na * na * (L + 1);
     if (opt.precision != kFp64) b += 4 * na * na * L;
     if (opt.precision != kFp64 && na >= 17 && na <= 1024) {
@@ -84,7 +86,8 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
 size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
-           Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + CT_.bytes() + DT_.bytes() +
+           Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + BfT64_.bytes() + CT_.bytes() +
+           DT_.bytes() +
            PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + Sg_.bytes() + Sks32_.bytes() +
            Sks64_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
            tot_.bytes() + seedNext_.bytes();
@@ -157,6 +160,15 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
         } else {
             if (m > 0) k::gemm_f64(s, n, n, m, A, B, AB, mm);
             k::abm_combine(s, false, n, m, h, A, A2, B, AB, MT, Bbar64_.get(), Bbar32_.get());
+            if (m > 0) {   // leaves as [B | AB] times two combinations of the samples
+                k::transpose<double>(s, n, m, B, BfT64_.get());
+                k::transpose<double>(s, n, m, AB, BfT64_.get() + (size_t)m * n);
+                k::LeafComb G;
+                k::abm_leaf_factors(h, G);
+                for (int q = 0; q < 5; ++q)
+                    for (int g = 0; g < 2; ++g) leafG_[q][g] = G.g[q][g];
+                leafFac_ = true;
+            }
         }
     }
     const int kq = method_ == kABM ? 5 : 3;
@@ -202,6 +214,32 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
     const int kq = method_ == kABM ? 5 : 3;
     const int off = method_ == kABM ? 3 * n_ : 0;
     if (nPad > nPadMax_) throw std::invalid_argument("ScanCore::leaves: batch too long");
+    if (leafFac_) {
+        const bool wz = xIsDouble() || !structured_level0(nReal, nPad);
+        k::LeafComb G;
+        for (int q = 0; q < 5; ++q)
+            for (int g = 0; g < 2; ++g) G.g[q][g] = leafG_[q][g];
+        bool done;
+        if (xIsDouble())
+            done = uDouble ? k::leaves_factored<double, double>(s, n_, m_, kq, 2, na_, off, nReal,
+                                                                nPad, BfT64_.get(), G,
+                                                                (const double*)dU, sL, oL, rs,
+                                                                X64_.get(), wz)
+                           : k::leaves_factored<double, float>(s, n_, m_, kq, 2, na_, off, nReal,
+                                                               nPad, BfT64_.get(), G,
+                                                               (const float*)dU, sL, oL, rs,
+                                                               X64_.get(), wz);
+        else
+            done = uDouble ? k::leaves_factored<float, double>(s, n_, m_, kq, 2, na_, off, nReal,
+                                                               nPad, BfT64_.get(), G,
+                                                               (const double*)dU, sL, oL, rs,
+                                                               X32_.get(), wz)
+                           : k::leaves_factored<float, float>(s, n_, m_, kq, 2, na_, off, nReal,
+                                                              nPad, BfT64_.get(), G,
+                                                              (const float*)dU, sL, oL, rs,
+                                                              X32_.get(), wz);
+        if (done) return;
+    }
     if (xIsDouble()) {
         if (uDouble)
             k::leaves<double, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
@@ -310,9 +348,11 @@ void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
     const int lt = top_level(nReal, nPad);
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
     for (int l = lt - 1; l >= (s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));
-    if (s0)   // the X[L] leaves of level 0 hold only their last n rows
+    // the X[L] leaves of level 0 hold only their last n rows, and the outputs
+    // read only the last n rows of each state: the rest is not computed
+    if (s0)
         k::level_mfma(s, true, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(), 0,
-                      3 * n_, Sg_.get());
+                      3 * n_, Sg_.get(), (3 * n_) / 32 * 32);
 }
 
 void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot,

@@ -103,6 +103,10 @@ private:
     DevArray<double> prod64_;     // FP64 products of A
     DevArray<double> A64_, B64_;
     DevArray<double> Bbar64_, BbT64_;
+    // ABM with an FP64 map: [B | AB] transposed and the factors of the leaves
+    DevArray<double> BfT64_;
+    double leafG_[5][2] = {};
+    bool leafFac_ = false;
     DevArray<float> Bbar32_, BbT32_;
     DevArray<float> CT_, DT_;        // C^T and D^T for the output stage
     // power table (transposed), L+1 levels, and FP32 level tables

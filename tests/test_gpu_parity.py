@@ -272,6 +272,33 @@ def test_precision_modes_against_fp64(solver, mode, bound):
     assert err <= max(bound, 1e-7)
 
 
+@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-5)])
+@pytest.mark.parametrize("solver,n,m,steps,f32", [(1, 17, 1, 1000, False), (1, 40, 3, 3001, True),
+                                                  (0, 5, 7, 999, False), (0, 70, 5, 2049, True),
+                                                  (1, 64, 8, 4097, False)])
+def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
+    """The FP64 leaves run on the matrix cores in 16 x 16 tiles: ragged row
+    tiles (n not a multiple of 16), input counts that do not fill a 4-column
+    k-step, ragged horizons, FP32 and FP64 samples, against the FP64 oracle."""
+    s = synth.make_system(solver, n, m, 2, -6.0, -0.05, 1e-3, steps, seed=n + m)
+    U = synth.samples_f64(s)
+    if f32:
+        U = U.astype(np.float32)
+    api.set_precision(mode)
+    try:
+        tv, y, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h,
+                                            s.x0)
+    finally:
+        api.set_precision(api.PLAIN32)
+    assert e == 0, api.last_error_message()
+    y64 = orc.simulate_f64(solver, s.A.astype(np.float64), s.B.astype(np.float64),
+                           s.C.astype(np.float64), s.D.astype(np.float64),
+                           s.x0.astype(np.float64), U.astype(np.float64), s.steps, s.h)
+    err = orc.max_rel_err(y, y64)
+    print(f"mode {mode} solver {solver} n {n} m {m}: max rel err vs FP64 {err:.3e}")
+    assert err <= bound
+
+
 # ------------------------------------------- matrix-core vs vector kernels
 @pytest.mark.parametrize("solver,n", [(0, 20), (0, 48), (0, 100), (0, 256), (0, 300), (1, 17),
                                       (1, 40), (1, 64), (1, 130), (1, 256)])
