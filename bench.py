@@ -33,16 +33,22 @@ TRAFFIC_LEVEL0_DOWN = (577050 * 2.0 + 1048574) * 1024
 
 def level_flops(na, n_scan, n_leaf=None):
     """Algorithmic flops of the level sweeps: 2 na^2 per pair, pairs of
-    level l = ceil(N / 2^(l+1)), both sweeps.  With n_leaf (ABM: the n
-    nonzero rows of each n_a-row leaf), level 0 costs 2 na n_leaf per pair in
-    each sweep: in the up-sweep only those source states are nonzero, in the
-    down-sweep only those rows of each state are read by the outputs."""
+    level l = ceil(N / 2^(l+1)), both sweeps.  With n_leaf (ABM, n = na / 4:
+    the companion map M has rows below 3n that are shifts by n, M^2 rows below
+    2n that are shifts by 2n, and the leaves are nonzero only in their last n
+    rows), the two lowest levels need fewer products per pair:
+      level 0: up 2 n n (M's last n rows times the leaf's last n rows),
+               down 2 n na (only the last n rows, which the outputs read)
+      level 1: up 2 (2n)(2n), down 2 (2n) na."""
     L = max(0, (n_scan - 1).bit_length())
     per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
     pairs = sum(per)
     flops = 2 * 2.0 * na * na * pairs
     if n_leaf and L:
-        flops -= 2 * 2.0 * na * (na - n_leaf) * per[0]
+        n = n_leaf
+        flops -= per[0] * (2 * 2.0 * na * na - 2.0 * n * n - 2.0 * n * na)
+        if L > 1:
+            flops -= per[1] * (2 * 2.0 * na * na - 2.0 * (2 * n) * (2 * n) - 2.0 * (2 * n) * na)
     return flops, pairs
 
 

@@ -86,16 +86,22 @@ bool level_mfma_supported(int na);
 int mfma_table_rows(int na);
 void make_level_table_padded(hipStream_t s, int na, int nap, int nrp, const double* P, bool delta,
                              float* PTp);
-// rbeg: first row that is not zero in the level-0 leaves (ABM: 3n; the rows
-// below are not read); jbeg: first source state the up-sweep product needs (a
-// multiple of 32, <= rbeg; 0 for the down-sweep).  Both 0 in general.
-// ibase: the down-sweep writes rows i >= ibase only (a multiple of 32, <= rbeg;
-// ABM level 0: the rows the outputs read); 0 in general and in the up-sweep.
+// Structure of the lowest ABM levels (all zero in general): rbeg, the first
+// row of the X[L] source / X[R] accumulator (up) or X[L] addend (down) that
+// is not zero (rows below are not read); jbeg, the first source state the
+// product needs (a multiple of 32, <= rbeg; 0 for the down-sweep); ibase, the
+// first row the product forms (a multiple of 32, <= rbeg); rows [clo, ibase)
+// are shifts of the source by sh rows, (P x)[i] = x[i + sh], and are copied
+// (the down-sweep also copies X[L] <- X[R] there); rows below clo are not
+// written; [zlo, zhi) (up only, <= ibase) rows of X[R] written as zero.
+struct LevelRows {
+    int jbeg = 0, rbeg = 0, ibase = 0, clo = 0, sh = 0, zlo = 0, zhi = 0;
+};
 // For more than 256 rows (na > 256: level_mfma_splits) the rows are split over
 // work-groups, and the down-sweep first copies the pairs' X[R] into S
 // (np * na floats).
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, int jbeg = 0, int rbeg = 0, float* S = nullptr, int ibase = 0);
+                float* X, const LevelRows& lr = LevelRows{}, float* S = nullptr);
 inline bool level_mfma_splits(int na) { return na > 256; }
 
 // A level with few pairs as many small K-split tiles (32 x 32 FP32 MFMA when

@@ -1165,25 +1165,22 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // PT is the zero-padded transposed table, [nrp][nap] with nap = 32*ceil(na/32)
 // and nrp a multiple of KC, so fragment loads need no bounds checks; the next
 // k-step's B fragments are loaded while the current MFMAs run.
-// Level 0 of the ABM scan reads leaves that hold only the last n of the 4n
-// companion rows (the leaves leave the zero rows unwritten): rbeg = 3n is the
-// first row of a leaf that is not zero (the X[L] source and the X[R]
-// accumulator of the up-sweep, the X[L] addend of the down-sweep), and jbeg =
-// 32 floor(3n / 32) the first source state the up-sweep product needs.
-// Elsewhere both are 0.
+// The lowest levels of the ABM scan have structure (LevelRows, all zero in
+// general):
+//   rbeg  first row of the X[L] source and X[R] accumulator (up) or the X[L]
+//         addend (down) that is not zero; rows below are never read
+//   jbeg  first source state the product needs (32 floor(rbeg / 32) up, 0 down)
+//   ibase first row formed by the product (a multiple of 32); rows [clo, ibase)
+//         are rows of P that are shifts, (P x)[i] = x[i + sh], and are copied;
+//         below clo nothing is written (nothing reads those rows)
+//   [zlo, zhi) rows of the up-sweep X[R] written as zero (the next level's
+//         source rows between its jbeg and rbeg)
+// Level 0 (P = M, leaves nonzero from 3n): rows below 3n of M are shifts by n.
+// Level 1 (P = M^2, inputs nonzero from 2n): rows below 2n are shifts by 2n.
 //
-// The down-sweep of that level may also produce only rows i >= ibase (a
-// multiple of 32; ABM: the rows the outputs read, the last n): the rows below
-// keep stale values, which nothing reads afterwards.  Elsewhere ibase is 0.
-//
-// For rows beyond one work-group's (IB = WGI*NIT*32), blockIdx.y picks a
-// block of IB rows i.  The up-sweep stays race-free (it reads X[L], writes
-// X[R]).  The down-sweep reads its source columns from S, a compact copy of
-// the old X[R] ([np][na], made by k_gather_r), because other row blocks write
-// X[R] while this one still reads it.  S is null when one block covers n_a.
 template <bool DOWN, bool DELTA, int NIT, int NCT, int WGI, int KC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, long d, long np,
-                                                    int jbeg, int rbeg, int ibase,
+                                                    LevelRows lr,
                                                     const float* __restrict__ PT,
                                                     const float* __restrict__ S,
                                                     float* __restrict__ X)
@@ -1198,6 +1195,7 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     const int wi = wave % WGI, wc = wave / WGI;
     const int l32 = lane & 31, hi = lane >> 5;
     const long k0 = (long)blockIdx.x * TC;
+    const int jbeg = lr.jbeg, rbeg = lr.rbeg, ibase = lr.ibase;
     const int ib0 = ibase + (SPLIT ? (int)blockIdx.y * IB : 0);
     constexpr bool fromS = DOWN && SPLIT;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
@@ -1300,6 +1298,27 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
         }
         __syncthreads();
     }
+    // shift rows [clo, ibase) and zero rows [zlo, zhi) of the tile's columns
+    // (one row block does them); a wave per column, lanes over rows
+    if ((lr.clo < ibase || lr.zlo < lr.zhi) && (!SPLIT || blockIdx.y == 0)) {
+        for (int c = wave; c < ncol; c += 4) {
+            const long R = (k0 + c + 1) * 2 * d - 1;
+            for (int i = lr.clo + lane; i < ibase; i += 64) {
+                if (!DOWN) {
+                    X[R * na + i] = X[(R - d) * na + i + lr.sh];
+                } else {
+                    const float xs = fromS ? S[(k0 + c) * na + i + lr.sh] : X[R * na + i + lr.sh];
+                    const float xr = fromS ? S[(k0 + c) * na + i] : X[R * na + i];
+                    X[(R - d) * na + i] = xr;
+                    X[R * na + i] = xs;
+                }
+            }
+            if (!DOWN)
+                for (int i = lr.zlo + lane; i < lr.zhi; i += 64) X[R * na + i] = 0.0f;
+        }
+        // the down-sweep copies read rows of X[R] that the epilogue overwrites
+        if (DOWN && !fromS) __syncthreads();
+    }
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
@@ -1361,7 +1380,7 @@ __global__ void k_gather_r(int na, long d, long np, const float* __restrict__ X,
 
 template <int NIT, int NCT, int WGI, bool SPLIT>
 static void mfma_launch_t(hipStream_t s, bool down, bool delta, int na, int nap, int nrp, long d,
-                          long np, int jbeg, int rbeg, int ibase, const float* PT, float* X,
+                          long np, const LevelRows& lr, const float* PT, float* X,
                           const float* S, dim3 g)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
@@ -1370,33 +1389,33 @@ static void mfma_launch_t(hipStream_t s, bool down, bool delta, int na, int nap,
     if (down) {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<true, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s, na,
-                               nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
+                               nap, nrp, d, np, lr, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<true, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
+                               na, nap, nrp, d, np, lr, PT, S, X);
     } else {
         if (delta)
             hipLaunchKernelGGL((k_level_mfma<false, true, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
+                               na, nap, nrp, d, np, lr, PT, S, X);
         else
             hipLaunchKernelGGL((k_level_mfma<false, false, NIT, NCT, WGI, KC, SPLIT>), g, b, 0, s,
-                               na, nap, nrp, d, np, jbeg, rbeg, ibase, PT, S, X);
+                               na, nap, nrp, d, np, lr, PT, S, X);
     }
     check_launch("level_mfma");
 }
 
 template <int NIT, int NCT, int WGI>
 static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, long np,
-                        int jbeg, int rbeg, int ibase, const float* PT, float* X, float* S)
+                        const LevelRows& lr, const float* PT, float* X, float* S)
 {
     constexpr int TC = (4 / WGI) * NCT * 32;
     constexpr int IB = WGI * NIT * 32;
     const int nap = (na + 31) / 32 * 32, nrp = mfma_table_rows(na);
-    const unsigned gy = (unsigned)((na - ibase + IB - 1) / IB);
+    const unsigned gy = (unsigned)((na - lr.ibase + IB - 1) / IB);
     dim3 g((unsigned)((np + TC - 1) / TC), gy);
     if (gy == 1) {
-        mfma_launch_t<NIT, NCT, WGI, false>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, ibase,
-                                            PT, X, nullptr, g);
+        mfma_launch_t<NIT, NCT, WGI, false>(s, down, delta, na, nap, nrp, d, np, lr, PT, X,
+                                            nullptr, g);
         return;
     }
     if (down) {
@@ -1405,8 +1424,7 @@ static void mfma_launch(hipStream_t s, bool down, bool delta, int na, long d, lo
                            d, np, X, S);
         check_launch("gather_r");
     }
-    mfma_launch_t<NIT, NCT, WGI, true>(s, down, delta, na, nap, nrp, d, np, jbeg, rbeg, ibase, PT,
-                                       X, S, g);
+    mfma_launch_t<NIT, NCT, WGI, true>(s, down, delta, na, nap, nrp, d, np, lr, PT, X, S, g);
 }
 
 // Rows of the padded MFMA table: a multiple of every K-slice (<= 256).
@@ -1417,30 +1435,35 @@ bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
 // Tile choice by scan size; the narrow variant (32 pairs per work-group)
 // takes over when the wide one would leave the 256 CUs short of work.
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, int jbeg, int rbeg, float* S, int ibase)
+                float* X, const LevelRows& lr, float* S)
 {
     if (np <= 0) return;
-    if (jbeg % 32 != 0 || jbeg < 0 || jbeg >= na || rbeg < 0 || rbeg >= na || jbeg > rbeg ||
-        (down && jbeg != 0) || (delta && rbeg != 0) || ibase % 32 != 0 || ibase < 0 ||
-        ibase >= na || ibase > rbeg || (!down && ibase != 0))
+    const bool shifts = lr.clo < lr.ibase, zeros = lr.zlo < lr.zhi;
+    if (lr.jbeg % 32 != 0 || lr.jbeg < 0 || lr.jbeg >= na || lr.rbeg < 0 || lr.rbeg >= na ||
+        lr.jbeg > lr.rbeg || (down && lr.jbeg != 0) || (delta && lr.rbeg != 0) ||
+        lr.ibase % 32 != 0 || lr.ibase < 0 || lr.ibase >= na || lr.ibase > lr.rbeg ||
+        lr.clo < 0 || lr.clo > lr.ibase || (shifts && (lr.sh <= 0 || lr.ibase + lr.sh > na)) ||
+        (zeros && (down || lr.zlo < 0 || lr.zhi > lr.ibase || (shifts && lr.zhi > lr.clo))) ||
+        (delta && (shifts || zeros)))
         throw std::invalid_argument("level_mfma: bad structured range");
+    const int ibase = lr.ibase;
     const int nt = (na - ibase + 31) / 32;
     const bool narrow = np < 2L * 256 * 64;
     if (nt <= 1) {
-        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
-        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, lr, PT, X, nullptr);
+        else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, lr, PT, X, nullptr);
     } else if (nt <= 2) {
-        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
-        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 2>(s, down, delta, na, d, np, lr, PT, X, nullptr);
+        else mfma_launch<1, 2, 2>(s, down, delta, na, d, np, lr, PT, X, nullptr);
     } else if (nt <= 4) {
-        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
-        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        if (narrow) mfma_launch<1, 1, 4>(s, down, delta, na, d, np, lr, PT, X, nullptr);
+        else mfma_launch<1, 2, 4>(s, down, delta, na, d, np, lr, PT, X, nullptr);
     } else if (nt <= 8) {
-        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
-        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, nullptr);
+        if (narrow) mfma_launch<2, 1, 4>(s, down, delta, na, d, np, lr, PT, X, nullptr);
+        else mfma_launch<2, 2, 4>(s, down, delta, na, d, np, lr, PT, X, nullptr);
     } else {
-        // above 256 states: 256-row blocks side by side (blockIdx.y)
-        mfma_launch<2, 2, 4>(s, down, delta, na, d, np, jbeg, rbeg, ibase, PT, X, S);
+        // above 256 rows: 256-row blocks side by side (blockIdx.y)
+        mfma_launch<2, 2, 4>(s, down, delta, na, d, np, lr, PT, X, S);
     }
 }
 

@@ -288,7 +288,7 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
     } else if ((level_kernel() & 1) == 0 && PTm_.get()) {
         k::level_mfma(s, down, level_is_delta(l), na_, d, np,
-                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(), 0, 0, Sg_.get());
+                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(), k::LevelRows{}, Sg_.get());
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
                                X32_.get());
@@ -312,14 +312,60 @@ int ScanCore::top_level(long nReal, long nPad) const
     return l;
 }
 
+// Level 1 of the ABM scan on the matrix cores with its inputs nonzero from
+// row 2n (level 0 leaves rows below 2n unwritten): rows below 2n of M^2 are
+// shifts by 2n, so the product forms only rows from 32 floor(2n / 32) on.
+bool ScanCore::structured_level1(long nReal, long nPad) const
+{
+    const int lt = top_level(nReal, nPad);
+    return structured_level0(nReal, nPad) && L_ > 1 && lt > 1 && !level_is_f64(1) &&
+           !level_is_delta(1) && (pairs_at(nReal, 1) > k::kKsMaxPairs || na_ < kKsMinStates);
+}
+
+// Rows of the structured ABM levels (M: rows below 3n are shifts by n; M^2:
+// rows below 2n are shifts by 2n).
+static k::LevelRows abm_rows(int n, int level, bool down, bool next)
+{
+    k::LevelRows r;
+    const int b0 = (3 * n) / 32 * 32, b1 = (2 * n) / 32 * 32;
+    if (level == 0) {
+        r.rbeg = 3 * n;
+        r.jbeg = down ? 0 : b0;
+        if (down) {            // only the rows the outputs read
+            r.ibase = b0;
+            r.clo = b0;
+        } else if (next) {     // rows from 2n for a structured level 1
+            r.ibase = b0;
+            r.clo = std::min(2 * n, b0);
+            r.sh = n;
+            r.zlo = std::min(b1, b0);
+            r.zhi = r.clo;
+        }
+    } else {
+        r.rbeg = 2 * n;
+        r.jbeg = down ? 0 : b1;
+        r.ibase = b1;
+        r.clo = 0;
+        r.sh = 2 * n;
+    }
+    return r;
+}
+
 void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
     int l = 0;
     if (lt > 0 && structured_level0(nReal, nPad)) {
+        const bool s1 = structured_level1(nReal, nPad);
         k::level_mfma(s, false, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(),
-                      (3 * n_) / 32 * 32, 3 * n_, Sg_.get());
+                      abm_rows(n_, 0, false, s1), Sg_.get());
         l = 1;
+        if (s1) {
+            k::level_mfma(s, false, false, na_, 2, pairs_at(nReal, 1),
+                          PTm_.get() + (size_t)nap_ * nrp_, X32_.get(), abm_rows(n_, 1, false, false),
+                          Sg_.get());
+            l = 2;
+        }
     }
     for (; l < lt; ++l) level(s, false, l, pairs_at(nReal, l));
 }
@@ -347,12 +393,16 @@ void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
-    for (int l = lt - 1; l >= (s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));
+    const bool s1 = s0 && structured_level1(nReal, nPad);
+    for (int l = lt - 1; l >= (s1 ? 2 : s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));
+    if (s1)
+        k::level_mfma(s, true, false, na_, 2, pairs_at(nReal, 1), PTm_.get() + (size_t)nap_ * nrp_,
+                      X32_.get(), abm_rows(n_, 1, true, false), Sg_.get());
     // the X[L] leaves of level 0 hold only their last n rows, and the outputs
     // read only the last n rows of each state: the rest is not computed
     if (s0)
-        k::level_mfma(s, true, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(), 0,
-                      3 * n_, Sg_.get(), (3 * n_) / 32 * 32);
+        k::level_mfma(s, true, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(),
+                      abm_rows(n_, 0, true, false), Sg_.get());
 }
 
 void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot,

@@ -87,6 +87,7 @@ public:
     void* X() const { return xIsDouble() ? (void*)X64_.get() : (void*)X32_.get(); }
 
     bool structured_level0(long nReal, long nPad) const;
+    bool structured_level1(long nReal, long nPad) const;
 
 private:
     bool level_is_f64(int l) const;
