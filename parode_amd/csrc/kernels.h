@@ -108,7 +108,7 @@ inline bool level_mfma_splits(int na) { return na > 256; }
 // TT = float, with PT the padded MFMA table and ldp its row length; 16 x 16
 // FP64 MFMA when TT = double, with PT the transposed FP64 power, ldp = na).
 // The down-sweep first copies the pairs' X[R] into S (np * na values).
-constexpr long kKsMaxPairs = 256;
+constexpr long kKsMaxPairs = 4096;
 template <typename TT, typename TX>
 void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
               TX* S);
@@ -133,6 +133,17 @@ template <typename TX, typename TU>
 void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
              const float* C, const float* D, const TX* X, const TU* U, long sO, long oO,
              float* Y);
+
+// The same outputs in FP64 with one thread per element (X or U in FP64), C^T
+// and D^T given in FP64 with rows padded with zeros to pp columns (a multiple
+// of 8; pad_cast makes them).  Returns false (nothing launched) when the shape
+// does not fit the kernel (states as outputs, or an element's state block too
+// large to stage).
+void pad_cast(hipStream_t s, int rows, int cols, int pp, const float* in, double* out);
+template <typename TX, typename TU>
+bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, bool zC, bool zD,
+                   long N, const double* CT, const double* DT, const TX* X, const TU* U, long sO,
+                   long oO, float* Y);
 
 // ---------------------------------------------------------- ABM start-up
 // X0 = [x0; x1; x2; x3] by three classic RK4 steps on Uh (9 rows, h/2 grid).

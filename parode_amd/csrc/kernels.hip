@@ -1947,6 +1947,125 @@ __global__ __launch_bounds__(256) void k_outputs_direct(int n, int m, int p, int
     }
 }
 
+// Outputs with one thread per element (a work-group of kOutLanes elements), in
+// FP64: the elements' state blocks are staged in LDS (coalesced rows, odd
+// pitch); C^T and D^T, FP64 with rows padded with zeros to pp (a multiple of
+// 8) columns, are read at addresses that are the same for every lane (scalar
+// loads of eight values), and each thread accumulates eight of its element's
+// outputs at a time in registers.  Sums over c in order, C terms then D terms,
+// as k_outputs.
+constexpr int kOutLanes = 64;
+
+template <typename TX, typename TU>
+__global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
+    int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+    const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
+    const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
+{
+    extern __shared__ unsigned char outl_smem[];
+    TX* xs = reinterpret_cast<TX*>(outl_smem);   // [kOutLanes][n + 1]
+    const int xp = n + 1;
+    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += (long)gridDim.x * kOutLanes) {
+        const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
+        __syncthreads();
+        // sixteen elements' rows in flight per thread
+        if (useC)
+            for (int c = threadIdx.x; c < n; c += kOutLanes)
+                for (int q0 = 0; q0 < cnt; q0 += 16) {
+                    TX v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const int q = q0 + u < cnt ? q0 + u : cnt - 1;
+                        v[u] = X[(e0 + q) * na + (long)oX * n + c];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        if (q0 + u < cnt) xs[(q0 + u) * xp + c] = v[u];
+                }
+        __syncthreads();
+        if ((int)threadIdx.x >= cnt) continue;
+        const long e = e0 + threadIdx.x;
+        const TX* x = xs + threadIdx.x * xp;
+        const TU* u = U + (e * sO + oO) * m;
+        float* y = Y + e * p;
+        for (int jb = 0; jb < p; jb += 8) {
+            double v[8];
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) v[jj] = 0.0;
+            if (useC)
+                for (int c = 0; c < n; ++c) {
+                    const double xc = (double)x[c];
+                    const double* cr = CT + (long)c * pp + jb;
+#pragma unroll
+                    for (int jj = 0; jj < 8; ++jj) v[jj] = fma(cr[jj], xc, v[jj]);
+                }
+            if (useD)
+                for (int c0 = 0; c0 < m; c0 += 8) {
+                    double uv[8];   // eight inputs in flight (indices past m repeat m - 1)
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) uv[cc] = (double)u[c0 + cc < m ? c0 + cc : m - 1];
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        if (c0 + cc >= m) break;
+                        const double* dr = DT + (long)(c0 + cc) * pp + jb;
+#pragma unroll
+                        for (int jj = 0; jj < 8; ++jj) v[jj] = fma(dr[jj], uv[cc], v[jj]);
+                    }
+                }
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj)
+                if (jb + jj < p) y[jb + jj] = (float)v[jj];
+        }
+    }
+}
+
+__global__ void k_pad_cast(int rows, int cols, int pp, const float* __restrict__ in,
+                           double* __restrict__ out)
+{
+    const int total = rows * pp;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int r = t / pp, c = t - r * pp;
+        out[t] = c < cols ? (double)in[(long)r * cols + c] : 0.0;
+    }
+}
+
+void pad_cast(hipStream_t s, int rows, int cols, int pp, const float* in, double* out)
+{
+    if (rows <= 0 || pp <= 0) return;
+    hipLaunchKernelGGL(k_pad_cast, dim3(blocks_for((long)rows * pp, 256, 1024)), dim3(256), 0, s,
+                       rows, cols, pp, in, out);
+    check_launch("pad_cast");
+}
+
+template <typename TX, typename TU>
+bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, bool zC, bool zD,
+                   long N, const double* CT, const double* DT, const TX* X, const TU* U, long sO,
+                   long oO, float* Y)
+{
+    const bool useC = !zC, useD = !zD && m > 0;
+    if (N <= 0 || p <= 0) return N <= 0;
+    if (zC && zD) return false;   // the outputs are the states: k_outputs copies them
+    if (pp % 8 != 0 || pp < p) throw std::invalid_argument("outputs_lanes: bad padding");
+    const size_t lds = (size_t)kOutLanes * (n + 1) * sizeof(TX);
+    if (lds > 65536) return false;
+    const long blocks = std::min<long>((N + kOutLanes - 1) / kOutLanes, 1L << 16);
+    hipLaunchKernelGGL((k_outputs_lanes<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)),
+                       dim3(kOutLanes), lds, s, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U,
+                       sO, oO, Y);
+    check_launch("outputs_lanes");
+    return true;
+}
+
+template bool outputs_lanes<float, double>(hipStream_t, int, int, int, int, int, int, bool, bool,
+                                           long, const double*, const double*, const float*,
+                                           const double*, long, long, float*);
+template bool outputs_lanes<double, double>(hipStream_t, int, int, int, int, int, int, bool,
+                                            bool, long, const double*, const double*,
+                                            const double*, const double*, long, long, float*);
+template bool outputs_lanes<double, float>(hipStream_t, int, int, int, int, int, int, bool, bool,
+                                           long, const double*, const double*, const double*,
+                                           const float*, long, long, float*);
+
 template <typename TX, typename TU>
 void outputs(hipStream_t s, int n, int m, int p, int na, int oX, bool zC, bool zD, long N,
              const float* C, const float* D, const TX* X, const TU* U, long sO, long oO, float* Y)

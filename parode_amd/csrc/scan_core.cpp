@@ -5,6 +5,11 @@
 #include "config.h"
 #include "kernels.h"
 
+namespace {
+// columns padded to a multiple of 8 (at least 8)
+int pad8(int c) { return c > 0 ? (c + 7) / 8 * 8 : 8; }
+}  // namespace
+
 namespace parode {
 
 ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadMax, ScanOptions opt,
@@ -30,6 +35,8 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     if (method == kABM && opt_.precision != kPlain32 && m > 0) BfT64_.alloc(2 * nm, arena);
     CT_.alloc((size_t)n * (p > 0 ? p : 1), arena);
     DT_.alloc((size_t)(m > 0 ? m : 1) * (p > 0 ? p : 1), arena);
+    CT64_.alloc((size_t)n * pad8(p), arena);
+    DT64_.alloc((size_t)(m > 0 ? m : 1) * pad8(p), arena);
     PT64_.alloc(na2 * (L_ + 1), arena);
     int n32 = 0;
     for (int l = 0; l < L_; ++l)
@@ -59,7 +66,8 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
     seedNext_.alloc(na_, arena);
 }
 
-size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt)
+size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt,
+                                    int p)
 {
     const long nP = next_pow2(nPadMax < 1 ? 1 : nPadMax);
     const int L = ilog2(nP);
@@ -69,6 +77,7 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     if (opt.precision != kPlain32) b += 8 * (nn + nm);
     b += 24 * (size_t)n * (method == kABM ? 5 : 3) * (m > 0 ? m : 1);
     if (method == kABM && opt.precision != kPlain32 && m > 0) b += 16 * nm;
+    b += (size_t)(n + (m > 0 ? m : 1)) * (4 * (p > 0 ? p : 1) + 8 * pad8(p));   // C^T, D^T
     b += 8 * na * na * (L + 1);
     if (opt.precision != kFp64) b += 4 * na * na * L;
     if (opt.precision != kFp64 && na >= 17 && na <= 1024) {
@@ -87,7 +96,7 @@ size_t ScanCore::footprint() const
 {
     return prod32_.bytes() + prod64_.bytes() + A64_.bytes() + B64_.bytes() + Bbar64_.bytes() +
            Bbar32_.bytes() + BbT64_.bytes() + BbT32_.bytes() + BfT64_.bytes() + CT_.bytes() +
-           DT_.bytes() +
+           DT_.bytes() + CT64_.bytes() + DT64_.bytes() +
            PT64_.bytes() + PT32_.bytes() + PTm_.bytes() + Sg_.bytes() + Sks32_.bytes() +
            Sks64_.bytes() + X32_.bytes() + X64_.bytes() + seed_.bytes() +
            tot_.bytes() + seedNext_.bytes();
@@ -421,6 +430,27 @@ void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const vo
     if (!zD && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
     dC = CT_.get();
     dD = DT_.get();
+    // FP64 outputs: one thread per element where the shape fits
+    if (xIsDouble() || uDouble) {
+        const int pp = pad8(p_);
+        if (!zC) k::pad_cast(s, n_, p_, pp, CT_.get(), CT64_.get());
+        if (!zD && m_ > 0) k::pad_cast(s, m_, p_, pp, DT_.get(), DT64_.get());
+        bool done;
+        if (xIsDouble())
+            done = uDouble ? k::outputs_lanes<double, double>(s, n_, m_, p_, pp, na_, oX, zC_, zD_,
+                                                               N, CT64_.get(), DT64_.get(),
+                                                               X64_.get(), (const double*)dU, sO,
+                                                               oO, dY)
+                           : k::outputs_lanes<double, float>(s, n_, m_, p_, pp, na_, oX, zC_, zD_,
+                                                              N, CT64_.get(), DT64_.get(),
+                                                              X64_.get(), (const float*)dU, sO, oO,
+                                                              dY);
+        else
+            done = k::outputs_lanes<float, double>(s, n_, m_, p_, pp, na_, oX, zC_, zD_, N,
+                                                   CT64_.get(), DT64_.get(), X32_.get(),
+                                                   (const double*)dU, sO, oO, dY);
+        if (done) return;
+    }
     if (xIsDouble()) {
         if (uDouble)
             k::outputs<double, double>(s, n_, m_, p_, na_, oX, zC_, zD_, N, dC, dD, X64_.get(),

@@ -47,7 +47,9 @@ public:
     const ScanOptions& options() const { return opt_; }
     bool xIsDouble() const { return opt_.precision == kFp64; }
     size_t footprint() const;
-    static size_t estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt);
+    // p: outputs (C and D, transposed in FP32 and FP64); 0 when not known
+    static size_t estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt,
+                                     int p = 0);
 
     void build_map(hipStream_t s, const float* dA, const float* dB, double h);
     void build_powers(hipStream_t s);
@@ -106,6 +108,7 @@ private:
     DevArray<double> Bbar64_, BbT64_;
     // ABM with an FP64 map: [B | AB] transposed and the factors of the leaves
     DevArray<double> BfT64_;
+    DevArray<double> CT64_, DT64_;   // C^T and D^T in FP64 for the outputs
     double leafG_[5][2] = {};
     bool leafFac_ = false;
     DevArray<float> Bbar32_, BbT32_;

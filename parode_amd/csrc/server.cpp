@@ -69,7 +69,7 @@ size_t Server::footprint(const ServeRequest& r, int precision)
     ScanOptions opt;
     opt.precision = precision;
     const long nScan = r.solver == kRK4 ? r.points : r.points - 3;
-    return ScanCore::estimate_footprint(r.solver, r.n, r.m, nScan, opt) + layout(r).ioBytes;
+    return ScanCore::estimate_footprint(r.solver, r.n, r.m, nScan, opt, r.p) + layout(r).ioBytes;
 }
 
 Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)

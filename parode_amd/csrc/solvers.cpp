@@ -345,7 +345,8 @@ void RungeKutta4thOrder::InitializeSolverData(const matrixf& A, const matrixf& B
     ScanOptions opt;
     opt.precision = _nPrecision;
     const size_t fixed = parode::ScanCore::estimate_footprint(parode::kRK4, (int)_nMatrixSize,
-                                                              (int)_nInputs, 1L << 20, opt) -
+                                                              (int)_nInputs, 1L << 20, opt,
+                                                              (int)_nOutputSize) -
                          (size_t)(1L << 20) * _nMatrixSize * xBytes;
     std::vector<unsigned long> per(1, PlanStepsPerBatch(_nSteps, perElem, fixed));
     SetNStepsPerDevice(per);
@@ -447,7 +448,8 @@ void AdamsMoulton::InitializeSolverData(const matrixf& A, const matrixf& B, doub
     ScanOptions opt;
     opt.precision = _nPrecision;
     const size_t fixed = parode::ScanCore::estimate_footprint(parode::kABM, (int)_nSystemSize,
-                                                              (int)_nInputs, 1L << 20, opt) -
+                                                              (int)_nInputs, 1L << 20, opt,
+                                                              (int)_nOutputSize) -
                          (size_t)(1L << 20) * _nMatrixSize * xBytes;
     std::vector<unsigned long> per(1, PlanStepsPerBatch(_nSteps, perElem, fixed));
     SetNStepsPerDevice(per);
