@@ -84,6 +84,18 @@ int ParODE_PlanCapture(void* plan, const float* dA, const float* dB, const float
 int ParODE_PlanReplay(void* plan, void* stream);
 long ParODE_PlanFootprint(void* plan);
 int ParODE_PlanLevels(void* plan);
+/* Autotuner: times each kernel that can run each scan level of this plan's
+ * shape (K-split tiles, narrow or wide matrix-core tiles, SIMT) on the plan's
+ * own trajectory buffer, on `stream`, and keeps the fastest in a process-wide
+ * level schedule that every later solve of the same shape follows (plans,
+ * the C ABI, served requests).  Returns the number of level launches tuned
+ * (>= 0) or a negative ErrorCode.  Blocks until the timings are done; the
+ * plan's previous outputs are not touched, its trajectory is overwritten.
+ * For callers that solve one shape many times.  No counterpart in the
+ * reference. */
+int ParODE_PlanAutotune(void* plan, void* stream);
+/* Forgets the autotuned schedule (every level back to the built-in rule). */
+void ParODE_ScheduleClear(void);
 int ParODE_PlanDestroy(void* plan);
 /* Device memory the plan for this shape would hold, without allocating. */
 long ParODE_PlanEstimateBytes(int solver, int nStates, int nInputs, long points, int precision);

@@ -21,7 +21,8 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_PlanCreate", "ParODE_PlanExecute", "ParODE_PlanExecuteTimed",
                "ParODE_PlanFootprint",
                "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes",
-               "ParODE_PlanCapture", "ParODE_PlanReplay",
+               "ParODE_PlanCapture", "ParODE_PlanReplay", "ParODE_PlanAutotune",
+               "ParODE_ScheduleClear",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
                "ParODE_ServerDestroy"]
@@ -94,6 +95,10 @@ def lib():
         L.ParODE_PlanFootprint.restype = ctypes.c_long
         L.ParODE_PlanLevels.argtypes = [_vp]
         L.ParODE_PlanLevels.restype = ctypes.c_int
+        L.ParODE_PlanAutotune.argtypes = [_vp, _vp]
+        L.ParODE_PlanAutotune.restype = ctypes.c_int
+        L.ParODE_ScheduleClear.argtypes = []
+        L.ParODE_ScheduleClear.restype = None
         L.ParODE_PlanDestroy.argtypes = [_vp]
         L.ParODE_PlanDestroy.restype = ctypes.c_int
         L.ParODE_PlanEstimateBytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -331,6 +336,14 @@ class Plan:
     def levels(self):
         return lib().ParODE_PlanLevels(self.handle)
 
+    def autotune(self, stream=0):
+        """Times the kernels that can run each scan level of this shape and
+        keeps the fastest (process-wide); returns the level launches tuned."""
+        r = lib().ParODE_PlanAutotune(self.handle, ctypes.c_void_p(stream))
+        if r < 0:
+            raise RuntimeError(f"ParODE_PlanAutotune: {r} {last_error_message()}")
+        return r
+
     def close(self):
         if self.handle:
             lib().ParODE_PlanDestroy(self.handle)
@@ -413,3 +426,8 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def schedule_clear():
+    """Every scan level back to the built-in kernel rule."""
+    lib().ParODE_ScheduleClear()

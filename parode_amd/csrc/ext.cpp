@@ -207,6 +207,21 @@ int ParODE_PlanLevels(void* plan)
     return pl ? pl->core->levels() : -1;
 }
 
+int ParODE_PlanAutotune(void* plan, void* stream)
+{
+    Plan* pl = static_cast<Plan*>(plan);
+    if (!pl) return ParODE_InvalidSystem;
+    try {
+        PARODE_HIP(hipSetDevice(pl->device));
+        return pl->core->autotune(static_cast<hipStream_t>(stream), pl->nScan, pl->nPad);
+    } catch (std::exception& e) {
+        parode::last_error() = e.what();
+        return ParODE_NotEnoughResources;
+    }
+}
+
+void ParODE_ScheduleClear(void) { parode::ScanCore::clear_schedule(); }
+
 int ParODE_PlanDestroy(void* plan)
 {
     Plan* pl = static_cast<Plan*>(plan);

@@ -25,6 +25,7 @@ template <typename T>
 void transpose(hipStream_t s, int rows, int cols, const T* in, T* out);
 void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out);
 void fill_f64(hipStream_t s, long count, double v, double* out);
+void fill_zero_f32(hipStream_t s, long count, float* out);
 
 // RK4 combination (RungeKutta4thOrder.cpp:48-57).  mirror: inputs are the float
 // products, combination in double without contraction, one rounding to float;
@@ -100,8 +101,10 @@ struct LevelRows {
 // For more than 256 rows (na > 256: level_mfma_splits) the rows are split over
 // work-groups, and the down-sweep first copies the pairs' X[R] into S
 // (np * na floats).
+// tile: 0 by the pair count, 1 the narrow tiles (32 pairs per work-group), 2
+// the wide ones.
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, const LevelRows& lr = LevelRows{}, float* S = nullptr);
+                float* X, const LevelRows& lr = LevelRows{}, float* S = nullptr, int tile = 0);
 inline bool level_mfma_splits(int na) { return na > 256; }
 
 // A level with few pairs as many small K-split tiles (32 x 32 FP32 MFMA when

@@ -302,6 +302,14 @@ __global__ void k_fill_f64(long count, double v, double* __restrict__ out)
         out[i] = v;
 }
 
+void fill_zero_f32(hipStream_t s, long count, float* out)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_fill<float>), dim3(blocks_for(count, 256, 8192)), dim3(256), 0, s, count,
+                       out);
+    check_launch("fill_zero_f32");
+}
+
 void fill_f64(hipStream_t s, long count, double v, double* out)
 {
     hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(count, 256, 4096)), dim3(256), 0, s, count, v,
@@ -1435,7 +1443,7 @@ bool level_mfma_supported(int na) { return na >= 17 && na <= 1024; }
 // Tile choice by scan size; the narrow variant (32 pairs per work-group)
 // takes over when the wide one would leave the 256 CUs short of work.
 void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, const float* PT,
-                float* X, const LevelRows& lr, float* S)
+                float* X, const LevelRows& lr, float* S, int tile)
 {
     if (np <= 0) return;
     const bool shifts = lr.clo < lr.ibase, zeros = lr.zlo < lr.zhi;
@@ -1448,7 +1456,7 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
         throw std::invalid_argument("level_mfma: bad structured range");
     const int ibase = lr.ibase;
     const int nt = (na - ibase + 31) / 32;
-    const bool narrow = np < 2L * 256 * 64;
+    const bool narrow = tile == 1 || (tile == 0 && np < 2L * 256 * 64);
     if (nt <= 1) {
         if (narrow) mfma_launch<1, 1, 1>(s, down, delta, na, d, np, lr, PT, X, nullptr);
         else mfma_launch<1, 2, 1>(s, down, delta, na, d, np, lr, PT, X, nullptr);

@@ -1,6 +1,9 @@
 #include "scan_core.h"
 
 #include <algorithm>
+#include <array>
+#include <mutex>
+#include <map>
 
 #include "config.h"
 #include "kernels.h"
@@ -268,12 +271,60 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
 }
 
 // --------------------------------------------------------------------- sweeps
+namespace {
+// The autotuned schedule: (n_a, precision, level, pairs, down) -> path
+std::mutex g_schedMu;
+std::map<std::array<long, 5>, int> g_sched;
+}  // namespace
+
+int ScanCore::scheduled_path(bool down, int l, long np) const
+{
+    std::lock_guard<std::mutex> lk(g_schedMu);
+    auto it = g_sched.find({(long)na_, (long)opt_.precision, (long)l, np, down ? 1L : 0L});
+    return it == g_sched.end() ? kPathAuto : it->second;
+}
+
+void ScanCore::clear_schedule()
+{
+    std::lock_guard<std::mutex> lk(g_schedMu);
+    g_sched.clear();
+}
+
 void ScanCore::level(hipStream_t s, bool down, int l, long np)
+{
+    level_with(s, down, l, np, scheduled_path(down, l, np));
+}
+
+// The kernels that can run level l with np pairs: K-split tiles (few pairs),
+// matrix-core tiles (FP32 tables), SIMT (FP64 tables).
+std::vector<int> ScanCore::level_candidates(int l, long np) const
+{
+    std::vector<int> c;
+    const bool ks = np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l);
+    const bool f64 = xIsDouble() || level_is_f64(l);
+    const bool mfma = !f64 && (level_kernel() & 1) == 0 && PTm_.get();
+    if (ks && (f64 || mfma)) c.push_back(kPathKs);
+    if (mfma) {
+        c.push_back(kPathNarrow);
+        c.push_back(kPathWide);
+    } else {
+        c.push_back(kPathSimt);
+    }
+    return c;
+}
+
+void ScanCore::level_with(hipStream_t s, bool down, int l, long np, int path)
 {
     const size_t na2 = (size_t)na_ * na_;
     const long d = 1L << l;
+    const bool ksOk = np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l);
+    const bool mfmaOk = !xIsDouble() && !level_is_f64(l) && (level_kernel() & 1) == 0 && PTm_.get();
+    int tile = 0;   // matrix-core tiles by the pair count
+    if (path == kPathAuto && ksOk) path = kPathKs;
+    if (path == kPathNarrow) tile = 1;
+    if (path == kPathWide) tile = 2;
     // few pairs: small K-split tiles on the matrix cores, spread over many CUs
-    if (np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l)) {
+    if (path == kPathKs && ksOk) {
         if (xIsDouble()) {
             k::level_ks<double, double>(s, down, na_, na_, d, np, PT64_.get() + l * na2,
                                         X64_.get(), Sks64_.get());
@@ -284,7 +335,7 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
                                        X32_.get(), Sks32_.get());
             return;
         }
-        if ((level_kernel() & 1) == 0 && PTm_.get()) {
+        if (mfmaOk) {
             k::level_ks<float, float>(s, down, na_, nap_, d, np,
                                       PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(),
                                       Sks32_.get());
@@ -295,9 +346,10 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
         k::level<double, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X64_.get());
     } else if (level_is_f64(l)) {
         k::level<float, double>(s, down, false, na_, d, np, PT64_.get() + l * na2, X32_.get());
-    } else if ((level_kernel() & 1) == 0 && PTm_.get()) {
+    } else if (mfmaOk) {
         k::level_mfma(s, down, level_is_delta(l), na_, d, np,
-                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(), k::LevelRows{}, Sg_.get());
+                      PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(), k::LevelRows{}, Sg_.get(),
+                      tile);
     } else {
         k::level<float, float>(s, down, level_is_delta(l), na_, d, np, PT32_.get() + l * na2,
                                X32_.get());
@@ -377,6 +429,59 @@ void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
         }
     }
     for (; l < lt; ++l) level(s, false, l, pairs_at(nReal, l));
+}
+
+int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
+{
+    if (nPad > nPadMax_ || nReal < 1) throw std::invalid_argument("ScanCore::autotune: bad length");
+    const int lt = top_level(nReal, nPad);
+    const bool s0 = lt > 0 && structured_level0(nReal, nPad);
+    const bool s1 = s0 && structured_level1(nReal, nPad);
+    // timings on zeros (the kernels' speed does not depend on the values)
+    if (xIsDouble())
+        k::fill_f64(s, nPad * na_, 0.0, X64_.get());
+    else
+        k::fill_zero_f32(s, nPad * na_, X32_.get());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PARODE_HIP(hipEventCreate(&e0));
+    PARODE_HIP(hipEventCreate(&e1));
+    int tuned = 0;
+    try {
+        for (int l = 0; l < lt; ++l) {
+            if ((l == 0 && s0) || (l == 1 && s1)) continue;   // fixed structured launches
+            const long np = pairs_at(nReal, l);
+            const std::vector<int> cand = level_candidates(l, np);
+            if (cand.size() < 2) continue;
+            for (int down = 0; down < 2; ++down) {
+                int best = kPathAuto;
+                float bestMs = 0.0f;
+                for (int c : cand) {
+                    constexpr int kReps = 3;
+                    level_with(s, down != 0, l, np, c);   // warm-up
+                    PARODE_HIP(hipEventRecord(e0, s));
+                    for (int r = 0; r < kReps; ++r) level_with(s, down != 0, l, np, c);
+                    PARODE_HIP(hipEventRecord(e1, s));
+                    PARODE_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    PARODE_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (best == kPathAuto || ms < bestMs) {
+                        best = c;
+                        bestMs = ms;
+                    }
+                }
+                std::lock_guard<std::mutex> lk(g_schedMu);
+                g_sched[{(long)na_, (long)opt_.precision, (long)l, np, (long)down}] = best;
+                ++tuned;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    PARODE_HIP(hipEventDestroy(e0));
+    PARODE_HIP(hipEventDestroy(e1));
+    return tuned;
 }
 
 void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSeed, double* dTot)

@@ -18,6 +18,8 @@
 #pragma once
 #include "hip_util.h"
 
+#include <vector>
+
 namespace parode {
 
 enum Method { kRK4 = 0, kABM = 1 };
@@ -91,10 +93,24 @@ public:
     bool structured_level0(long nReal, long nPad) const;
     bool structured_level1(long nReal, long nPad) const;
 
+    // Autotuner: times every kernel that can run each level of this shape
+    // (K-split tiles, narrow or wide matrix-core tiles, SIMT) on this core's
+    // own trajectory buffer (its contents are overwritten) and keeps the
+    // fastest per (n_a, precision, level, pairs, sweep) in a process-wide
+    // schedule that every later solve of that shape follows.  Returns the
+    // number of level launches tuned.  Synchronises the stream.
+    int autotune(hipStream_t s, long nReal, long nPad);
+    // Level paths of the schedule (kPathAuto: the built-in rule)
+    enum LevelPath { kPathAuto = 0, kPathKs = 1, kPathNarrow = 2, kPathWide = 3, kPathSimt = 4 };
+    int scheduled_path(bool down, int l, long np) const;
+    static void clear_schedule();
+
 private:
     bool level_is_f64(int l) const;
     bool level_is_delta(int l) const;
     void level(hipStream_t s, bool down, int l, long np);
+    void level_with(hipStream_t s, bool down, int l, long np, int path);
+    std::vector<int> level_candidates(int l, long np) const;
 
     int method_, n_, m_, p_, na_;
     bool zC_, zD_;

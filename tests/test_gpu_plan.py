@@ -104,3 +104,48 @@ def test_plan_graph_replay_equals_eager():
         plan.execute(*ptrs, st.cuda_stream)      # eager work between replays
         torch.cuda.synchronize()
     plan.close()
+
+
+@pytest.mark.parametrize("solver,n,precision", [(1, 40, api.MIXED), (0, 70, api.PLAIN32),
+                                                (0, 24, api.FP64)])
+def test_autotuned_schedule_meets_the_bars(solver, n, precision):
+    """The autotuner times each level's kernels on the plan's own buffers and
+    the later solves follow its choices; results stay within the bars (the
+    choices change only the order of FP32 sums)."""
+    s = synth.make_system(solver, n, 3, 2, -6.0, -0.05, 1e-3, 20000, seed=n)
+    U = synth.samples_f64(s)
+    if precision == api.PLAIN32:
+        U = U.astype(np.float32)
+    plan = api.Plan(s.method, s.n, s.m, s.p, 0, 0, s.steps, s.h, precision,
+                    U.dtype == np.float64)
+    tA, tB, tC, tD, tx = (_dev(v) for v in (s.A, s.B, s.C, s.D, s.x0))
+    tU = _dev(U)
+    tY = torch.zeros((s.steps, s.p), dtype=torch.float32, device="cuda:0")
+    args = (tA.data_ptr(), tB.data_ptr(), tC.data_ptr(), tD.data_ptr(), tx.data_ptr(),
+            tU.data_ptr(), tY.data_ptr(), 0)
+    try:
+        plan.execute(*args)
+        torch.cuda.synchronize()
+        y0 = tY.cpu().numpy()
+        tuned = plan.autotune()
+        assert tuned > 0
+        tY.zero_()
+        plan.execute(*args)
+        torch.cuda.synchronize()
+        y1 = tY.cpu().numpy()
+    finally:
+        plan.close()
+        api.schedule_clear()
+    if precision == api.PLAIN32:
+        ye = orc.simulate_mirror_exact(solver, s.A, s.B, s.C, s.D, s.x0, 0.0, s.t_end, s.h,
+                                       Uhalf=U)
+        bound = 1e-5
+    else:
+        ye = orc.simulate_f64(solver, s.A.astype(np.float64), s.B.astype(np.float64),
+                              s.C.astype(np.float64), s.D.astype(np.float64),
+                              s.x0.astype(np.float64), U, s.steps, s.h)
+        bound = 1e-5 if precision == api.MIXED else 1e-7
+    print(f"levels tuned {tuned}: err before {orc.max_rel_err(y0, ye):.3e} "
+          f"after {orc.max_rel_err(y1, ye):.3e}")
+    assert orc.max_rel_err(y0, ye) <= bound
+    assert orc.max_rel_err(y1, ye) <= bound
