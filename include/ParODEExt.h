@@ -27,7 +27,8 @@ void ParODE_SetMaxBatchSteps(long maxSteps);
 /* Kernel choices, for A/B checks and tuning (0 = automatic): bit 0 runs the
  * FP32 scan levels on the vector ALUs instead of the matrix cores, bit 1 the
  * FP64 GEMMs of the power table on the vector ALUs, bit 2 launches every level
- * separately instead of finishing the top of the tree in one work-group. */
+ * separately instead of finishing the top of the tree in one work-group, bit 3
+ * runs the lowest ABM levels as tree levels instead of chunk recurrences. */
 void ParODE_SetLevelKernel(int variant);
 
 /* Compiles a UFunc source for a gfx target (e.g. "gfx950") without a GPU.

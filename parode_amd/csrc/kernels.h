@@ -107,6 +107,19 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
                 float* X, const LevelRows& lr = LevelRows{}, float* S = nullptr, int tile = 0);
 inline bool level_mfma_splits(int na) { return na > 256; }
 
+// The lowest lc levels of the ABM scan (n <= 64) as a sequential recurrence
+// per chunk of 2^lc elements on the matrix cores, in delta form with a
+// compensated newest block; P64 is the FP64 transposed map (P64[k][i] =
+// M[i][k], n_a x n_a).  Pass 1 (pass2 = false, in place of the
+// up-sweep's levels 0..lc-1) writes each chunk's aggregate to its last
+// element; pass 2 (in place of the down-sweep's levels lc-1..0) starts from
+// the state the down-sweep leaves there and writes the last block of every
+// real element's state, X[e][3n:4n), for the outputs.  Reads the leaves'
+// rows [3n, 4n).
+bool abm_chunks_supported(int n);
+void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
+                const double* P64, float* X);
+
 // A level with few pairs as many small K-split tiles (32 x 32 FP32 MFMA when
 // TT = float, with PT the padded MFMA table and ldp its row length; 16 x 16
 // FP64 MFMA when TT = double, with PT the transposed FP64 power, ldp = na).

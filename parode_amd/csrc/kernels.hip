@@ -797,44 +797,84 @@ __global__ __launch_bounds__(256) void k_leaves_mfma(int n, int m, int kq, int n
     const int pw = km | 1;                                  // W pitch (FAC)
     double* Ws = Us + stagedMax * pitch;                    // perBlock x pw (FAC)
     const int nz = na - n;
-    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
+    // sample values of block eb staged into LDS (rows t / m by multiply-high)
+    auto block_vals = [&](long eb) {
         const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
-        const long row0 = eb * sL + oL;
         long rows = (long)(cnt - 1) * sL + (long)(kq - 1) * rs + 1;
+        const long row0 = eb * sL + oL;
         if (rows > lastRow - row0 + 1) rows = lastRow - row0 + 1;
-        const int vals = rows > 0 ? (int)(rows * m) : 0;
-        const TU* src = U + row0 * m;
-        // stage (eight loads in flight per thread); row t / m by multiply-high
-        constexpr int SU = 8;
-        for (int t0 = 0; t0 < vals; t0 += 256 * SU) {
-            double v[SU];
+        return rows > 0 ? (int)(rows * m) : 0;
+    };
+    constexpr int SU = 8;
+    // When one block's samples fit SU values per thread, the next block's are
+    // loaded into registers while this block computes.
+    const bool ahead = stagedMax * m <= 256L * SU;
+    double v[SU];
+    auto load_regs = [&](long eb) {
+        const int vals = block_vals(eb);
+        const TU* src = U + (eb * sL + oL) * m;
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int t = t0 + threadIdx.x + u * 256;
-                v[u] = t < vals ? (double)src[t] : 0.0;
-            }
+        for (int u = 0; u < SU; ++u) {
+            const int t = threadIdx.x + u * 256;
+            v[u] = t < vals ? (double)src[t] : 0.0;
+        }
+    };
+    auto store_regs = [&](int t0, int vals) {
 #pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int t = t0 + threadIdx.x + u * 256;
-                const int r = m == 1 ? t : (int)__umulhi((unsigned)t, magic);
-                if (t < vals) Us[r * pitch + (t - r * m)] = v[u];
+        for (int u = 0; u < SU; ++u) {
+            const int t = t0 + threadIdx.x + u * 256;
+            const int r = m == 1 ? t : (int)__umulhi((unsigned)t, magic);
+            if (t < vals) Us[r * pitch + (t - r * m)] = v[u];
+        }
+    };
+    const long stride = (long)gridDim.x * perBlock;
+    if (ahead && (long)blockIdx.x * perBlock < nPad) load_regs((long)blockIdx.x * perBlock);
+    // When every wave has one row tile for the whole launch and K fits eight
+    // k-steps, its B fragments stay in registers across blocks.
+    const bool keepB = rowTiles * groups <= 4 && ks <= 8;
+    double bk[8];
+    if (keepB && wave < rowTiles * groups) {
+        const int r0 = (wave % rowTiles) * 16;
+        const int rr = (r0 + l16) < n ? r0 + l16 : n - 1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int c = 4 * u + k4;
+            bk[u] = c < km ? BbT[(long)c * n + rr] : 0.0;
+        }
+    }
+    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += stride) {
+        const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
+        const int vals = block_vals(eb);
+        if (ahead) {
+            store_regs(0, vals);
+        } else {
+            // eight loads in flight per thread
+            for (int t0 = 0; t0 < vals; t0 += 256 * SU) {
+                const TU* src = U + (eb * sL + oL) * m;
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const int t = t0 + threadIdx.x + u * 256;
+                    v[u] = t < vals ? (double)src[t] : 0.0;
+                }
+                store_regs(t0, vals);
             }
         }
+        const int creal = nReal - eb < cnt ? (int)(nReal - eb > 0 ? nReal - eb : 0) : cnt;
         // zero rows [0, off) and [off + n, na); when the first level reads only
         // rows from zlo on, rows [zlo, off) of real elements and every zero row
         // of padding elements
         if (nz > 0) {
-            for (int le = 0; le < cnt; ++le) {
+            const int zfirst = (writeZeros || zlo < off) ? 0 : creal;
+            for (int le = zfirst; le < cnt; ++le) {
                 T* xe = X + (eb + le) * na;
-                const bool all = writeZeros || eb + le >= nReal;
+                const bool all = writeZeros || le >= creal;
                 const int zb = all ? 0 : zlo, ze = all ? nz : off;
                 for (int z = zb + (int)threadIdx.x; z < ze; z += 256)
                     xe[z < off ? z : z + n] = (T)0;
             }
         }
         __syncthreads();
-        // the real leaves of the block (the others are stored as zero)
-        const int creal = nReal - eb < cnt ? (int)(nReal - eb > 0 ? nReal - eb : 0) : cnt;
+        if (ahead && eb + stride < nPad) load_regs(eb + stride);
         if (FAC) {
             // W[le][g m + j] = sum_q G[q][g] u[le*sL + q*rs][j], q in order
             for (int t = threadIdx.x; t < creal * m; t += 256) {
@@ -871,10 +911,15 @@ __global__ __launch_bounds__(256) void k_leaves_mfma(int n, int m, int kq, int n
                 while (j >= m) { j -= m; ++q; }
             for (int kb = 0; kb < ks; kb += 8) {
                 double b[8];
+                if (keepB) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int c = 4 * (kb + u) + k4;
-                    b[u] = c < km ? BbT[(long)c * n + rr] : 0.0;
+                    for (int u = 0; u < 8; ++u) b[u] = bk[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int c = 4 * (kb + u) + k4;
+                        b[u] = c < km ? BbT[(long)c * n + rr] : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
@@ -936,7 +981,9 @@ static bool leaves_mfma(hipStream_t s, int n, int m, int kq, int na, int off, lo
     if (lds > 65536 || m >= 65536 || lds / sizeof(double) >= 65536 || perBlock * m >= 65536)
         return false;
     const unsigned magic = (unsigned)(((1ULL << 32) + m - 1) / m);
-    const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16);
+    // persistent work-groups (several blocks each, the next block's samples
+    // loaded during the current one)
+    const long blocks = std::min<long>((nPad + perBlock - 1) / perBlock, 2048);
     const dim3 grid((unsigned)std::max<long>(blocks, 1));
     if (kg > 0)
         hipLaunchKernelGGL((k_leaves_mfma<T, TU, true>), grid, dim3(256), lds, s, n, m, kq, na,
@@ -1475,6 +1522,189 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
     }
 }
 
+// ------------------------------------------------ ABM chunk recurrences
+// The lowest lc levels of the ABM scan, replaced by a sequential recurrence
+// per chunk of Lc = 2^lc elements.  The companion map shifts the state by one
+// block per step, so a step needs only the n x 4n dense block row of M:
+//   x_new[3n:4n) = x[3n:4n) + (M[3n:4n, :] - [0 0 0 I]) x + v_e[3n:4n),
+//   x_new[0:3n)  = x[n:4n).
+// The product is taken in delta form (the table is formed from the FP64 M,
+// minus the identity, then rounded once), and the running sum of the newest
+// block is compensated (Kahan): neither the rounding of M nor that of the
+// state accumulates over the chunk.
+// A work-group owns 32 chunks; its state vectors live in LDS as a ring of four
+// blocks (the shift is a change of the ring's head), and each wave forms one
+// 32-row tile of the new block on the matrix cores (v_mfma_f32_32x32x2f32):
+//   A (32 chunks x 2 states): the ring, from LDS (lane l: chunk l%32)
+//   B (2 states x 32 rows):   the delta table, held in registers for all k
+//   D: lane l, register v holds chunk 8(v/4) + 4(l/32) + v%4, row l%32
+// so each lane also keeps its 16 entries of the newest block and their
+// compensation terms in registers.  Blocks are padded to NP rows (the padded
+// rows of the table and of the leaves are zero, so they stay zero).
+//   Pass 1 (from zero): after Lc steps the ring is the chunk's aggregate, the
+//   value the up-sweep leaves at X[(c+1)Lc - 1]; it is written there.
+//   Pass 2 (after the down-sweep of the levels above): X[(c+1)Lc - 1] holds
+//   x at the chunk's first element; the steps produce x for the others, and
+//   the last block of each x_e is written to X[e][3n:4n) for the outputs.
+//   Element e's leaf v_e is read from X[e] before x_e is written there.
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 2) void k_abm_chunks(int n, int lc, long nReal, long nChunks,
+                                                       const double* __restrict__ P64,
+                                                       float* __restrict__ X)
+{
+#pragma clang fp contract(off)
+    constexpr int KS = 2 * NP;            // k-steps of two states over the 4 NP padded states
+    constexpr int PITCH = 4 * NP + 1;     // odd: the 32 chunks of a fragment in distinct banks
+    extern __shared__ float ring[];       // [32][PITCH]
+    const int na = 4 * n, Lc = 1 << lc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int r = wave * 32 + l32;        // row of the new block this lane forms
+    const bool rowOk = r < n;
+    const long c0 = (long)blockIdx.x * 32;
+    // B fragments: k-step ks covers padded states 2 ks + {0, 1}; padded state
+    // b NP + rr is the real state b n + rr when rr < n.  P64[k][i] = M[i][k].
+    float bm[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int kp = 2 * ks + hi, b = kp / NP, rr = kp % NP;
+        double m = 0.0;
+        if (rowOk && rr < n) m = P64[(long)(b * n + rr) * na + 3 * n + r] - (b == 3 && rr == r ? 1.0 : 0.0);
+        bm[ks] = (float)m;
+    }
+    // chunks of the work-group that hold real elements
+    long cReal = (nReal + Lc - 1) / Lc - c0;
+    cReal = cReal < 32 ? cReal : 32;
+    if (cReal <= 0) return;
+    // initial ring: zeros (pass 1) or the chunk's carry (pass 2), block b at slot b
+    for (int t = threadIdx.x; t < 32 * 4 * NP; t += NP * 2) {
+        const int c = t / (4 * NP), k = t % (4 * NP), b = k / NP, rr = k % NP;
+        float v = 0.0f;
+        if (PASS2 && c < cReal && rr < n) v = X[((c0 + c + 1) * Lc - 1) * na + b * n + rr];
+        ring[c * PITCH + k] = v;
+    }
+    const int steps = PASS2 ? Lc - 1 : Lc;
+    // leaves of the first step
+    float vl[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+        const int cc = c < cReal ? c : (int)cReal - 1;
+        vl[v] = rowOk ? X[((c0 + cc) * Lc) * na + 3 * n + r] : 0.0f;
+    }
+    __syncthreads();   // the ring is complete and the first leaves are in registers
+    // the newest block and its compensation (the true value is xh - xc)
+    float xh[16], xc[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+        xh[v] = rowOk ? ring[c * PITCH + 3 * NP + r] : 0.0f;
+        xc[v] = 0.0f;
+    }
+    // pass 2: the state of the chunk's first element goes out (over its leaf)
+    if (PASS2 && rowOk)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+            const long e = (c0 + c) * Lc;
+            if (c < cReal && e < nReal) X[e * na + 3 * n + r] = xh[v];
+        }
+    int head = 0;                         // slot of the oldest block
+    for (int j = 0; j < steps; ++j) {
+        floatx16 acc0, acc1;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            acc0[v] = 0.0f;
+            acc1[v] = 0.0f;
+        }
+        const float* arow = ring + l32 * PITCH + hi;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* ab = arow + ((head + b) & 3) * NP;
+#pragma unroll
+            for (int q = 0; q < NP / 2; q += 2) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[2 * q], bm[b * (NP / 2) + q], acc0,
+                                                            0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[2 * q + 2],
+                                                            bm[b * (NP / 2) + q + 1], acc1, 0, 0, 0);
+            }
+        }
+        // next step's leaves (element e + 1 of every chunk), read before the
+        // state x_{e+1} is written over them (pass 2)
+        float vn[16];
+        const bool more = j + 1 < steps || !PASS2;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+            const int cc = c < cReal ? c : (int)cReal - 1;
+            const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
+            vn[v] = (rowOk && more) ? X[((c0 + cc) * Lc + jn) * na + 3 * n + r] : 0.0f;
+        }
+        __syncthreads();                  // every wave has read the ring
+        const int slot = head;            // the oldest block becomes the newest
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+            // Kahan: xh + (delta + leaf), the lost low part kept in xc
+            const float y = (acc0[v] + acc1[v] + vl[v]) - xc[v];
+            const float sum = xh[v] + y;
+            xc[v] = (sum - xh[v]) - y;
+            xh[v] = sum;
+            if (rowOk) ring[c * PITCH + slot * NP + r] = sum;
+            if (PASS2 && rowOk) {
+                const long e = (c0 + c) * Lc + j + 1;
+                if (c < cReal && e < nReal) X[e * na + 3 * n + r] = sum - xc[v];
+            }
+            vl[v] = vn[v];
+        }
+        head = (head + 1) & 3;
+        __syncthreads();
+    }
+    if (PASS2) return;
+    // pass 1: the aggregate (ring, oldest block first; the newest block with
+    // its compensation) to the chunk's last element
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+        if (rowOk) ring[c * PITCH + ((head + 3) & 3) * NP + r] = xh[v] - xc[v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 32 * 4 * NP; t += NP * 2) {
+        const int c = t / (4 * NP), k = t % (4 * NP), b = k / NP, rr = k % NP;
+        if (c < cReal && rr < n)
+            X[((c0 + c + 1) * Lc - 1) * na + b * n + rr] = ring[c * PITCH + ((head + b) & 3) * NP + rr];
+    }
+}
+
+bool abm_chunks_supported(int n) { return n >= 1 && n <= 64; }
+
+void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
+                const double* P64, float* X)
+{
+    if (!abm_chunks_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
+        throw std::invalid_argument("abm_chunks: bad shape");
+    const long nChunks = nPad >> lc;
+    const dim3 grid((unsigned)((nChunks + 31) / 32));
+    if (n <= 32) {
+        const size_t lds = 32 * (4 * 32 + 1) * sizeof(float);
+        if (pass2)
+            hipLaunchKernelGGL((k_abm_chunks<32, true>), grid, dim3(64), lds, s, n, lc, nReal,
+                               nChunks, P64, X);
+        else
+            hipLaunchKernelGGL((k_abm_chunks<32, false>), grid, dim3(64), lds, s, n, lc, nReal,
+                               nChunks, P64, X);
+    } else {
+        const size_t lds = 32 * (4 * 64 + 1) * sizeof(float);
+        if (pass2)
+            hipLaunchKernelGGL((k_abm_chunks<64, true>), grid, dim3(128), lds, s, n, lc, nReal,
+                               nChunks, P64, X);
+        else
+            hipLaunchKernelGGL((k_abm_chunks<64, false>), grid, dim3(128), lds, s, n, lc, nReal,
+                               nChunks, P64, X);
+    }
+    check_launch("abm_chunks");
+}
+
 // ------------------------------------------- levels with few pairs (K split)
 // A work-group computes one tile of pairs x rows (32 x 32 on FP32 MFMA,
 // 16 x 16 on FP64 MFMA); its four waves split the K range (the source states)
@@ -1973,24 +2203,47 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
     extern __shared__ unsigned char outl_smem[];
     TX* xs = reinterpret_cast<TX*>(outl_smem);   // [kOutLanes][n + 1]
     const int xp = n + 1;
-    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += (long)gridDim.x * kOutLanes) {
+    const long stride = (long)gridDim.x * kOutLanes;
+    // With n <= kOutLanes, thread c holds state c of every element of a block:
+    // the next block's states are loaded into registers while this one computes.
+    // (FP32 states only: FP64 ones would take twice the registers)
+    constexpr int XV = sizeof(TX) == 4 ? kOutLanes : 16;
+    const bool ahead = useC && n <= kOutLanes && sizeof(TX) == 4;
+    TX xv[XV];
+    auto load_x = [&](long e0) {
+        const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
+        const int c = threadIdx.x < n ? (int)threadIdx.x : n - 1;
+#pragma unroll
+        for (int u = 0; u < XV; ++u) {
+            const int q = u < cnt ? u : cnt - 1;
+            xv[u] = X[(e0 + q) * na + (long)oX * n + c];
+        }
+    };
+    if (ahead && (long)blockIdx.x * kOutLanes < N) load_x((long)blockIdx.x * kOutLanes);
+    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += stride) {
         const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
         __syncthreads();
-        // sixteen elements' rows in flight per thread
-        if (useC)
+        if (ahead) {
+            if ((int)threadIdx.x < n)
+#pragma unroll
+                for (int u = 0; u < XV; ++u)
+                    if (u < cnt) xs[u * xp + threadIdx.x] = xv[u];
+        } else if (useC) {
+            // sixteen elements' rows in flight per thread
             for (int c = threadIdx.x; c < n; c += kOutLanes)
                 for (int q0 = 0; q0 < cnt; q0 += 16) {
-                    TX v[16];
 #pragma unroll
                     for (int u = 0; u < 16; ++u) {
                         const int q = q0 + u < cnt ? q0 + u : cnt - 1;
-                        v[u] = X[(e0 + q) * na + (long)oX * n + c];
+                        xv[u] = X[(e0 + q) * na + (long)oX * n + c];
                     }
 #pragma unroll
                     for (int u = 0; u < 16; ++u)
-                        if (q0 + u < cnt) xs[(q0 + u) * xp + c] = v[u];
+                        if (q0 + u < cnt) xs[(q0 + u) * xp + c] = xv[u];
                 }
+        }
         __syncthreads();
+        if (ahead && e0 + stride < N) load_x(e0 + stride);
         if ((int)threadIdx.x >= cnt) continue;
         const long e = e0 + threadIdx.x;
         const TX* x = xs + threadIdx.x * xp;
@@ -2056,7 +2309,8 @@ bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, b
     if (pp % 8 != 0 || pp < p) throw std::invalid_argument("outputs_lanes: bad padding");
     const size_t lds = (size_t)kOutLanes * (n + 1) * sizeof(TX);
     if (lds > 65536) return false;
-    const long blocks = std::min<long>((N + kOutLanes - 1) / kOutLanes, 1L << 16);
+    // persistent work-groups (several blocks each)
+    const long blocks = std::min<long>((N + kOutLanes - 1) / kOutLanes, 4096);
     hipLaunchKernelGGL((k_outputs_lanes<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)),
                        dim3(kOutLanes), lds, s, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U,
                        sO, oO, Y);

@@ -413,7 +413,7 @@ void ParODE_SetPrecision(int mode)
 
 int ParODE_GetPrecision(void) { return ParODE::Instance()->GetPrecision(); }
 
-void ParODE_SetLevelKernel(int variant) { parode::level_kernel() = variant & 7; }
+void ParODE_SetLevelKernel(int variant) { parode::level_kernel() = variant & 15; }
 
 void ParODE_SetMaxBatchSteps(long maxSteps) { parode::max_batch_steps() = maxSteps > 0 ? maxSteps : 0; }
 

@@ -217,7 +217,23 @@ void ScanCore::build_powers(hipStream_t s)
 bool ScanCore::structured_level0(long nReal, long nPad) const
 {
     return method_ == kABM && !xIsDouble() && L_ > 0 && !level_is_f64(0) && !level_is_delta(0) &&
-           (level_kernel() & 1) == 0 && PTm_.get() != nullptr && top_level(nReal, nPad) > 0;
+           (level_kernel() & 1) == 0 && PTm_.get() != nullptr && top_level(nReal, nPad) > 0 &&
+           chunk_log(nReal, nPad) == 0;
+}
+
+// ABM with n <= 64 and FP32 lowest levels: levels 0..5 run as recurrences over
+// chunks of 64 elements (the companion map needs only its n x 4n dense block
+// per step), when at least one tree level is launched above them.
+int ScanCore::chunk_log(long nReal, long nPad) const
+{
+    constexpr int kChunkLog = 6;
+    if (method_ != kABM || xIsDouble() || !k::abm_chunks_supported(n_) || !PTm_.get() ||
+        (level_kernel() & 9) != 0)
+        return 0;
+    for (int l = 0; l < kChunkLog; ++l)
+        if (level_is_f64(l) || level_is_delta(l)) return 0;
+    if (ilog2(nPad) <= kChunkLog || top_level(nReal, nPad) <= kChunkLog) return 0;
+    return kChunkLog;
 }
 
 void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs,
@@ -227,7 +243,8 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
     const int off = method_ == kABM ? 3 * n_ : 0;
     if (nPad > nPadMax_) throw std::invalid_argument("ScanCore::leaves: batch too long");
     if (leafFac_) {
-        const bool wz = xIsDouble() || !structured_level0(nReal, nPad);
+        const bool wz =
+            xIsDouble() || !(structured_level0(nReal, nPad) || chunk_log(nReal, nPad) > 0);
         k::LeafComb G;
         for (int q = 0; q < 5; ++q)
             for (int g = 0; g < 2; ++g) G.g[q][g] = leafG_[q][g];
@@ -262,11 +279,13 @@ void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long
     } else if (uDouble) {
         k::leaves<float, double, double>(s, n_, m_, kq, na_, off, nReal, nPad, BbT64_.get(),
                                          (const double*)dU, sL, oL, rs, X32_.get(),
-                                         !structured_level0(nReal, nPad));
+                                         !(structured_level0(nReal, nPad) ||
+                                           chunk_log(nReal, nPad) > 0));
     } else {
         k::leaves<float, float, float>(s, n_, m_, kq, na_, off, nReal, nPad, BbT32_.get(),
                                        (const float*)dU, sL, oL, rs, X32_.get(),
-                                       !structured_level0(nReal, nPad));
+                                       !(structured_level0(nReal, nPad) ||
+                                         chunk_log(nReal, nPad) > 0));
     }
 }
 
@@ -416,7 +435,10 @@ void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
     int l = 0;
-    if (lt > 0 && structured_level0(nReal, nPad)) {
+    if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
+        k::abm_chunks(s, false, n_, lc, nReal, nPad, PT64_.get(), X32_.get());
+        l = lc;
+    } else if (lt > 0 && structured_level0(nReal, nPad)) {
         const bool s1 = structured_level1(nReal, nPad);
         k::level_mfma(s, false, false, na_, 1, pairs_at(nReal, 0), PTm_.get(), X32_.get(),
                       abm_rows(n_, 0, false, s1), Sg_.get());
@@ -437,6 +459,7 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
     const int lt = top_level(nReal, nPad);
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
     const bool s1 = s0 && structured_level1(nReal, nPad);
+    const int lc = chunk_log(nReal, nPad);
     // timings on zeros (the kernels' speed does not depend on the values)
     if (xIsDouble())
         k::fill_f64(s, nPad * na_, 0.0, X64_.get());
@@ -448,7 +471,7 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
     int tuned = 0;
     try {
         for (int l = 0; l < lt; ++l) {
-            if ((l == 0 && s0) || (l == 1 && s1)) continue;   // fixed structured launches
+            if ((l == 0 && s0) || (l == 1 && s1) || l < lc) continue;   // fixed launches
             const long np = pairs_at(nReal, l);
             const std::vector<int> cand = level_candidates(l, np);
             if (cand.size() < 2) continue;
@@ -506,6 +529,11 @@ void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSe
 void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
+    if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
+        for (int l = lt - 1; l >= lc; --l) level(s, true, l, pairs_at(nReal, l));
+        k::abm_chunks(s, true, n_, lc, nReal, nPad, PT64_.get(), X32_.get());
+        return;
+    }
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
     const bool s1 = s0 && structured_level1(nReal, nPad);
     for (int l = lt - 1; l >= (s1 ? 2 : s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));

@@ -327,7 +327,7 @@ def test_mfma_levels_match_vector_levels(solver, n):
     assert e_mfma <= PARITY and e_vec <= PARITY
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4, 3, 5, 6])
+@pytest.mark.parametrize("variant", [1, 2, 4, 3, 5, 6, 8])
 @pytest.mark.parametrize("solver", [0, 1])
 def test_each_kernel_choice_meets_parity(solver, variant):
     """Every combination of the kernel switches (vector/MFMA levels, vector/MFMA
