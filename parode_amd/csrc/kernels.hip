@@ -10,6 +10,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 
@@ -1532,119 +1533,126 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
 // minus the identity, then rounded once), and the running sum of the newest
 // block is compensated (Kahan): neither the rounding of M nor that of the
 // state accumulates over the chunk.
-// A work-group owns 32 chunks; its state vectors live in LDS as a ring of four
-// blocks (the shift is a change of the ring's head), and each wave forms one
-// 32-row tile of the new block on the matrix cores (v_mfma_f32_32x32x2f32):
-//   A (32 chunks x 2 states): the ring, from LDS (lane l: chunk l%32)
-//   B (2 states x 32 rows):   the delta table, held in registers for all k
-//   D: lane l, register v holds chunk 8(v/4) + 4(l/32) + v%4, row l%32
-// so each lane also keeps its 16 entries of the newest block and their
-// compensation terms in registers.  Blocks are padded to NP rows (the padded
-// rows of the table and of the leaves are zero, so they stay zero).
+// A work-group owns 16 chunks; their state vectors live in LDS as a ring of
+// four blocks (the shift is a change of the ring's head), and each wave forms
+// one 16-row tile of the new block on the matrix cores (v_mfma_f32_16x16x4f32):
+//   A (16 chunks x 4 states): the ring, from LDS (lane l: chunk l%16)
+//   B (4 states x 16 rows):   the delta table, held in registers for all k
+//   D: lane l, register v holds chunk 4(l/16) + v, row l%16
+// K-step ks = b NP/4 + q covers padded states b NP + q + g NP/4 (g = l/16),
+// so a lane's A values of consecutive k-steps are adjacent in LDS.  Each lane
+// also keeps its 4 entries of the newest block and their compensation terms
+// in registers.  Blocks are padded to NP rows (the padded rows of the table
+// and of the leaves are zero, so they stay zero).
 //   Pass 1 (from zero): after Lc steps the ring is the chunk's aggregate, the
 //   value the up-sweep leaves at X[(c+1)Lc - 1]; it is written there.
 //   Pass 2 (after the down-sweep of the levels above): X[(c+1)Lc - 1] holds
 //   x at the chunk's first element; the steps produce x for the others, and
 //   the last block of each x_e is written to X[e][3n:4n) for the outputs.
 //   Element e's leaf v_e is read from X[e] before x_e is written there.
+constexpr int kChunkCols = 16;   // chunks per work-group
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
 template <int NP, bool PASS2>
-__global__ __launch_bounds__(NP * 2) void k_abm_chunks(int n, int lc, long nReal, long nChunks,
+__global__ __launch_bounds__(NP * 4) void k_abm_chunks(int n, int lc, long nReal, long nChunks,
                                                        const double* __restrict__ P64,
                                                        float* __restrict__ X)
 {
 #pragma clang fp contract(off)
-    constexpr int KS = 2 * NP;            // k-steps of two states over the 4 NP padded states
-    constexpr int PITCH = 4 * NP + 1;     // odd: the 32 chunks of a fragment in distinct banks
-    extern __shared__ float ring[];       // [32][PITCH]
+    constexpr int CH = kChunkCols;
+    constexpr int KS = NP;                // k-steps of four states over the 4 NP padded states
+    constexpr int PITCH = 4 * NP + 4;     // 4 mod 64 words; 16-byte aligned rows
+    constexpr int NT = NP * 4;            // threads
+    extern __shared__ float ring[];       // [CH][PITCH]
     const int na = 4 * n, Lc = 1 << lc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l32 = lane & 31, hi = lane >> 5;
-    const int r = wave * 32 + l32;        // row of the new block this lane forms
+    const int l16 = lane & 15, g = lane >> 4;
+    const int r = wave * 16 + l16;        // row of the new block this lane forms
     const bool rowOk = r < n;
-    const long c0 = (long)blockIdx.x * 32;
-    // B fragments: k-step ks covers padded states 2 ks + {0, 1}; padded state
-    // b NP + rr is the real state b n + rr when rr < n.  P64[k][i] = M[i][k].
+    const long c0 = (long)blockIdx.x * CH;
+    // B fragments (P64[k][i] = M[i][k]; padded state b NP + rr is the real
+    // state b n + rr when rr < n)
     float bm[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-        const int kp = 2 * ks + hi, b = kp / NP, rr = kp % NP;
+        const int b = ks / (NP / 4), rr = ks % (NP / 4) + g * (NP / 4);
         double m = 0.0;
         if (rowOk && rr < n) m = P64[(long)(b * n + rr) * na + 3 * n + r] - (b == 3 && rr == r ? 1.0 : 0.0);
         bm[ks] = (float)m;
     }
     // chunks of the work-group that hold real elements
     long cReal = (nReal + Lc - 1) / Lc - c0;
-    cReal = cReal < 32 ? cReal : 32;
+    cReal = cReal < CH ? cReal : CH;
     if (cReal <= 0) return;
     // initial ring: zeros (pass 1) or the chunk's carry (pass 2), block b at slot b
-    for (int t = threadIdx.x; t < 32 * 4 * NP; t += NP * 2) {
+    for (int t = threadIdx.x; t < CH * 4 * NP; t += NT) {
         const int c = t / (4 * NP), k = t % (4 * NP), b = k / NP, rr = k % NP;
         float v = 0.0f;
         if (PASS2 && c < cReal && rr < n) v = X[((c0 + c + 1) * Lc - 1) * na + b * n + rr];
         ring[c * PITCH + k] = v;
     }
     const int steps = PASS2 ? Lc - 1 : Lc;
-    // leaves of the first step
-    float vl[16];
+    // this lane's chunks (clamped to real ones for the loads)
+    int cl[4];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
-        const int cc = c < cReal ? c : (int)cReal - 1;
-        vl[v] = rowOk ? X[((c0 + cc) * Lc) * na + 3 * n + r] : 0.0f;
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * g + v;
+        cl[v] = c < cReal ? c : (int)cReal - 1;
     }
+    // leaves of the first step
+    float vl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) vl[v] = rowOk ? X[((c0 + cl[v]) * Lc) * na + 3 * n + r] : 0.0f;
     __syncthreads();   // the ring is complete and the first leaves are in registers
     // the newest block and its compensation (the true value is xh - xc)
-    float xh[16], xc[16];
+    float xh[4], xc[4];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
-        xh[v] = rowOk ? ring[c * PITCH + 3 * NP + r] : 0.0f;
+    for (int v = 0; v < 4; ++v) {
+        xh[v] = rowOk ? ring[(4 * g + v) * PITCH + 3 * NP + r] : 0.0f;
         xc[v] = 0.0f;
     }
     // pass 2: the state of the chunk's first element goes out (over its leaf)
     if (PASS2 && rowOk)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
             const long e = (c0 + c) * Lc;
             if (c < cReal && e < nReal) X[e * na + 3 * n + r] = xh[v];
         }
     int head = 0;                         // slot of the oldest block
     for (int j = 0; j < steps; ++j) {
-        floatx16 acc0, acc1;
+        // next step's leaves (element e + 1 of every chunk), issued before
+        // the products; read before the state x_{e+1} is written over them
+        float vn[4];
+        const bool more = j + 1 < steps || !PASS2;
+        const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            acc0[v] = 0.0f;
-            acc1[v] = 0.0f;
-        }
-        const float* arow = ring + l32 * PITCH + hi;
+        for (int v = 0; v < 4; ++v)
+            vn[v] = (rowOk && more) ? X[((c0 + cl[v]) * Lc + jn) * na + 3 * n + r] : 0.0f;
+        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float* arow = ring + l16 * PITCH + g * (NP / 4);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* ab = arow + ((head + b) & 3) * NP;
+            // a block's A values first (16-byte LDS reads), then its products
+            const floatx4* ab = reinterpret_cast<const floatx4*>(arow + ((head + b) & 3) * NP);
+            floatx4 a[NP / 16];
 #pragma unroll
-            for (int q = 0; q < NP / 2; q += 2) {
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[2 * q], bm[b * (NP / 2) + q], acc0,
-                                                            0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ab[2 * q + 2],
-                                                            bm[b * (NP / 2) + q + 1], acc1, 0, 0, 0);
+            for (int q = 0; q < NP / 16; ++q) a[q] = ab[q];
+#pragma unroll
+            for (int q = 0; q < NP / 16; ++q) {
+                const int ks = b * (NP / 4) + 4 * q;
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][0], bm[ks], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][1], bm[ks + 1], acc1, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][2], bm[ks + 2], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][3], bm[ks + 3], acc1, 0, 0, 0);
             }
-        }
-        // next step's leaves (element e + 1 of every chunk), read before the
-        // state x_{e+1} is written over them (pass 2)
-        float vn[16];
-        const bool more = j + 1 < steps || !PASS2;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
-            const int cc = c < cReal ? c : (int)cReal - 1;
-            const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
-            vn[v] = (rowOk && more) ? X[((c0 + cc) * Lc + jn) * na + 3 * n + r] : 0.0f;
         }
         __syncthreads();                  // every wave has read the ring
         const int slot = head;            // the oldest block becomes the newest
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
             // Kahan: xh + (delta + leaf), the lost low part kept in xc
             const float y = (acc0[v] + acc1[v] + vl[v]) - xc[v];
             const float sum = xh[v] + y;
@@ -1664,12 +1672,10 @@ __global__ __launch_bounds__(NP * 2) void k_abm_chunks(int n, int lc, long nReal
     // pass 1: the aggregate (ring, oldest block first; the newest block with
     // its compensation) to the chunk's last element
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int c = 8 * (v >> 2) + 4 * hi + (v & 3);
-        if (rowOk) ring[c * PITCH + ((head + 3) & 3) * NP + r] = xh[v] - xc[v];
-    }
+    for (int v = 0; v < 4; ++v)
+        if (rowOk) ring[(4 * g + v) * PITCH + ((head + 3) & 3) * NP + r] = xh[v] - xc[v];
     __syncthreads();
-    for (int t = threadIdx.x; t < 32 * 4 * NP; t += NP * 2) {
+    for (int t = threadIdx.x; t < CH * 4 * NP; t += NT) {
         const int c = t / (4 * NP), k = t % (4 * NP), b = k / NP, rr = k % NP;
         if (c < cReal && rr < n)
             X[((c0 + c + 1) * Lc - 1) * na + b * n + rr] = ring[c * PITCH + ((head + b) & 3) * NP + rr];
@@ -1684,24 +1690,23 @@ void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
     if (!abm_chunks_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
         throw std::invalid_argument("abm_chunks: bad shape");
     const long nChunks = nPad >> lc;
-    const dim3 grid((unsigned)((nChunks + 31) / 32));
-    if (n <= 32) {
-        const size_t lds = 32 * (4 * 32 + 1) * sizeof(float);
+    const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
+    auto launch = [&](auto np) {
+        constexpr int NP = decltype(np)::value;
+        const size_t lds = (size_t)kChunkCols * (4 * NP + 4) * sizeof(float);
         if (pass2)
-            hipLaunchKernelGGL((k_abm_chunks<32, true>), grid, dim3(64), lds, s, n, lc, nReal,
+            hipLaunchKernelGGL((k_abm_chunks<NP, true>), grid, dim3(NP * 4), lds, s, n, lc, nReal,
                                nChunks, P64, X);
         else
-            hipLaunchKernelGGL((k_abm_chunks<32, false>), grid, dim3(64), lds, s, n, lc, nReal,
+            hipLaunchKernelGGL((k_abm_chunks<NP, false>), grid, dim3(NP * 4), lds, s, n, lc, nReal,
                                nChunks, P64, X);
-    } else {
-        const size_t lds = 32 * (4 * 64 + 1) * sizeof(float);
-        if (pass2)
-            hipLaunchKernelGGL((k_abm_chunks<64, true>), grid, dim3(128), lds, s, n, lc, nReal,
-                               nChunks, P64, X);
-        else
-            hipLaunchKernelGGL((k_abm_chunks<64, false>), grid, dim3(128), lds, s, n, lc, nReal,
-                               nChunks, P64, X);
-    }
+    };
+    if (n <= 16)
+        launch(std::integral_constant<int, 16>());
+    else if (n <= 32)
+        launch(std::integral_constant<int, 32>());
+    else
+        launch(std::integral_constant<int, 64>());
     check_launch("abm_chunks");
 }
 
