@@ -24,24 +24,31 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 
-# HBM bytes of the largest launch of the level kernel at the default workload,
-# level 0 of the down-sweep (2^19 pairs of 256-float columns), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE counters, FETCH scaled by 2.0 (what the counter
-# reads for a plain copy of known size): (577050 * 2.0 + 1048574) KiB.
-TRAFFIC_LEVEL0_DOWN = (577050 * 2.0 + 1048574) * 1024
+# HBM bytes of the largest launch of the scan at the default workload, the
+# second chunk pass (k_abm_chunks, 2^20 elements in chunks of 64), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE counters, FETCH scaled by 2.0 (what the
+# counter reads for a plain copy of known size): (137808 * 2.0 + 262737) KiB.
+TRAFFIC_CHUNK_PASS2 = (137808 * 2.0 + 262737) * 1024
 
 
-def level_flops(na, n_scan, n_leaf=None):
-    """Algorithmic flops of the level sweeps: 2 na^2 per pair, pairs of
-    level l = ceil(N / 2^(l+1)), both sweeps.  With n_leaf (ABM, n = na / 4:
-    the companion map M has rows below 3n that are shifts by n, M^2 rows below
-    2n that are shifts by 2n, and the leaves are nonzero only in their last n
-    rows), the two lowest levels need fewer products per pair:
+def level_flops(na, n_scan, n_leaf=None, chunk_log=0):
+    """Algorithmic flops of the scan (both sweeps): 2 na^2 per pair, pairs of
+    level l = ceil(N / 2^(l+1)).  With n_leaf (ABM, n = na / 4: the companion
+    map M has rows below 3n that are shifts by n, M^2 rows below 2n that are
+    shifts by 2n, and the leaves are nonzero only in their last n rows), the
+    two lowest levels need fewer products per pair:
       level 0: up 2 n n (M's last n rows times the leaf's last n rows),
                down 2 n na (only the last n rows, which the outputs read)
-      level 1: up 2 (2n)(2n), down 2 (2n) na."""
+      level 1: up 2 (2n)(2n), down 2 (2n) na.
+    With chunk_log = lc (ABM chunk recurrences), levels below lc are replaced
+    by two passes of 2 n na per element (the n x 4n dense block of M per step),
+    and only levels >= lc run as tree levels.  Returns (flops, tree pairs)."""
     L = max(0, (n_scan - 1).bit_length())
     per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
+    if chunk_log and n_leaf:
+        n = n_leaf
+        tree = per[chunk_log:]
+        return 2 * 2.0 * n * na * n_scan + 2 * 2.0 * na * na * sum(tree), sum(tree)
     pairs = sum(per)
     flops = 2 * 2.0 * na * na * pairs
     if n_leaf and L:
@@ -334,7 +341,8 @@ def main():
 
     na = s.n_scan
     n_scan = s.steps if s.method == 0 else s.steps - 3
-    flops, pairs = level_flops(na, n_scan, s.n if s.method == 1 else None)
+    lc = plan.chunk_log
+    flops, pairs = level_flops(na, n_scan, s.n if s.method == 1 else None, lc)
     sweep_ms = ph["up_sweep"] + ph["down_sweep"]
     achieved = flops / (sweep_ms * 1e-3)
     peak = PEAK["fp64"] if prec == api.FP64 else PEAK["fp32"]
@@ -358,15 +366,17 @@ def main():
                    "precision": args.precision, "points": s.steps, "scan_size": na},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": TRAFFIC_LEVEL0_DOWN if (args.config == 1 and not args.points)
-                     else None,
-                     "traffic_launch": "level 0 of the down-sweep",
-                     "kernel": "k_level_mfma (every level launch of both sweeps)",
-                     "flops_per_solve": flops, "pairs_per_sweep": pairs, "sweep_ms": sweep_ms,
-                     # read X[R] and the leaf rows of X[L] that are not zero, write
-                     # the rows of both that the outputs read
-                     "algorithmic_bytes_level0_down":
-                         (n_scan // 2) * 4 * (na + 3 * s.n if s.method == 1 else 4 * na)},
+                     "traffic": TRAFFIC_CHUNK_PASS2 if (args.config == 1 and not args.points
+                                                        and lc == 6) else None,
+                     "traffic_launch": "second chunk pass (k_abm_chunks)" if lc else None,
+                     "kernel": ("k_abm_chunks (both passes) and the tree levels above them"
+                                if lc else "k_level_mfma (every level launch of both sweeps)"),
+                     "flops_per_solve": flops, "tree_pairs_per_sweep": pairs,
+                     "sweep_ms": sweep_ms, "chunk_log": lc,
+                     # second chunk pass: read each element's leaf block and each
+                     # chunk's carry, write each element's state block
+                     "algorithmic_bytes_traffic_launch":
+                         (n_scan * 8 * s.n + (n_scan >> lc) * 4 * na) if lc else None},
         "phases_ms": ph,
         "autotuned_level_launches": tuned,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},

@@ -22,7 +22,7 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_PlanFootprint",
                "ParODE_PlanLevels", "ParODE_PlanDestroy", "ParODE_PlanEstimateBytes",
                "ParODE_PlanCapture", "ParODE_PlanReplay", "ParODE_PlanAutotune",
-               "ParODE_ScheduleClear",
+               "ParODE_ScheduleClear", "ParODE_PlanChunkLog",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
                "ParODE_ServerDestroy"]
@@ -99,6 +99,8 @@ def lib():
         L.ParODE_PlanAutotune.restype = ctypes.c_int
         L.ParODE_ScheduleClear.argtypes = []
         L.ParODE_ScheduleClear.restype = None
+        L.ParODE_PlanChunkLog.argtypes = [_vp]
+        L.ParODE_PlanChunkLog.restype = ctypes.c_int
         L.ParODE_PlanDestroy.argtypes = [_vp]
         L.ParODE_PlanDestroy.restype = ctypes.c_int
         L.ParODE_PlanEstimateBytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -335,6 +337,11 @@ class Plan:
     @property
     def levels(self):
         return lib().ParODE_PlanLevels(self.handle)
+
+    @property
+    def chunk_log(self):
+        """log2 of the chunk length of the ABM chunk recurrences (0: none)."""
+        return lib().ParODE_PlanChunkLog(self.handle)
 
     def autotune(self, stream=0):
         """Times the kernels that can run each scan level of this shape and

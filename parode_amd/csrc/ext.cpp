@@ -222,6 +222,12 @@ int ParODE_PlanAutotune(void* plan, void* stream)
 
 void ParODE_ScheduleClear(void) { parode::ScanCore::clear_schedule(); }
 
+int ParODE_PlanChunkLog(void* plan)
+{
+    Plan* pl = static_cast<Plan*>(plan);
+    return pl ? pl->core->chunk_log(pl->nScan, pl->nPad) : -1;
+}
+
 int ParODE_PlanDestroy(void* plan)
 {
     Plan* pl = static_cast<Plan*>(plan);
