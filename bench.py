@@ -40,15 +40,19 @@ def level_flops(na, n_scan, n_leaf=None, chunk_log=0):
       level 0: up 2 n n (M's last n rows times the leaf's last n rows),
                down 2 n na (only the last n rows, which the outputs read)
       level 1: up 2 (2n)(2n), down 2 (2n) na.
-    With chunk_log = lc (ABM chunk recurrences), levels below lc are replaced
-    by two passes of 2 n na per element (the n x 4n dense block of M per step),
-    and only levels >= lc run as tree levels.  Returns (flops, tree pairs)."""
+    With chunk_log = lc (chunk recurrences), levels below lc are replaced by
+    two passes per element (ABM: 2 n na, the n x 4n dense block of M per step;
+    RK4: 2 na^2), and only levels >= lc run as tree levels.
+    Returns (flops, tree pairs)."""
     L = max(0, (n_scan - 1).bit_length())
     per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
-    if chunk_log and n_leaf:
-        n = n_leaf
+    if chunk_log:
+        # levels below chunk_log (ABM: the n x 4n block per step; RK4: the
+        # dense map), two passes over every element
+        per_step = 2.0 * n_leaf * na if n_leaf else 2.0 * na * na
+        flops = 2 * per_step * n_scan
         tree = per[chunk_log:]
-        return 2 * 2.0 * n * na * n_scan + 2 * 2.0 * na * na * sum(tree), sum(tree)
+        return flops + 2 * 2.0 * na * na * sum(tree), sum(tree)
     pairs = sum(per)
     flops = 2 * 2.0 * na * na * pairs
     if n_leaf and L:

@@ -97,9 +97,9 @@ int ParODE_PlanLevels(void* plan);
 int ParODE_PlanAutotune(void* plan, void* stream);
 /* Forgets the autotuned schedule (every level back to the built-in rule). */
 void ParODE_ScheduleClear(void);
-/* log2 of the chunk length when the plan's lowest ABM scan levels run as
- * chunk recurrences (0: as tree levels), -1 for a null plan.  For benchmarks
- * that count the work of each path. */
+/* Number of the plan's lowest scan levels that run as chunk recurrences
+ * (0: all run as tree levels), -1 for a null plan.  For benchmarks that count
+ * the work of each path. */
 int ParODE_PlanChunkLog(void* plan);
 int ParODE_PlanDestroy(void* plan);
 /* Device memory the plan for this shape would hold, without allocating. */

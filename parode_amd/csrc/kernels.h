@@ -116,6 +116,14 @@ inline bool level_mfma_splits(int na) { return na > 256; }
 // the state the down-sweep leaves there and writes the last block of every
 // real element's state, X[e][3n:4n), for the outputs.  Reads the leaves'
 // rows [3n, 4n).
+// Chunk recurrences of a dense map P (FP64 transposed, P64[k][i] = P[i][k],
+// na x na, na <= 256) over elements at X rows e S + S - 1, e < nElem (nReal
+// of them real), chunks of 2^lc elements: pass 1 writes each chunk's aggregate
+// to its last element, pass 2 writes every element's state from the value
+// there.  RK4 (S = 1, P = M) and the ABM chunk aggregates (S = 2^l, P = M^S).
+bool chunks_dense_supported(int na);
+void chunks_dense(hipStream_t s, bool pass2, int na, int lc, long S, long nReal, long nElem,
+                  const double* P64, float* X);
 bool abm_chunks_supported(int n);
 void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                 const double* P64, float* X);

@@ -1682,6 +1682,153 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks(int n, int lc, long nReal
     }
 }
 
+// Chunk recurrences of a dense map (RK4's M, or the ABM power P_l over chunk
+// aggregates): x_new = x + (P - I) x + v_e over the whole state, for elements
+// at X rows pos(e) = e S + S - 1 (S = 1: every element; S = 2^l: the values
+// the tree holds after level l - 1).  As in k_abm_chunks: a work-group owns 16
+// chunks with their states in LDS, a wave forms one 16-row tile of the new
+// state on v_mfma_f32_16x16x4f32 with its rows of P - I (formed from the FP64
+// table, then rounded once) in registers, and every state entry is a
+// compensated running sum.  Lane group g = l/16 takes the states
+// [g NP/4, (g+1) NP/4) of each k-step, so its A values are adjacent in LDS.
+//   Pass 1 (from zero) writes each chunk's aggregate to its last element's
+//   row; pass 2 starts from the value there (the state at the chunk's first
+//   element) and writes the state of every element of the chunk to its row,
+//   each after reading that element's leaf.
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void k_chunks_dense(int na, int lc, long S, long nReal,
+                                                         const double* __restrict__ P64,
+                                                         float* __restrict__ X)
+{
+#pragma clang fp contract(off)
+    constexpr int CH = kChunkCols;
+    constexpr int KS = NP / 4;            // k-steps of four states
+    constexpr int PITCH = NP + 4;         // 16-byte aligned rows, 16 chunks in distinct banks
+    constexpr int NT = NP * 4;
+    extern __shared__ float xsd[];        // [CH][PITCH]
+    const int Lc = 1 << lc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int r = wave * 16 + l16;
+    const bool rowOk = r < na;
+    const long c0 = (long)blockIdx.x * CH;
+    auto pos = [&](long e) { return e * S + S - 1; };
+    float bm[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int st = g * KS + ks;
+        double m = 0.0;
+        if (rowOk && st < na) m = P64[(long)st * na + r] - (st == r ? 1.0 : 0.0);
+        bm[ks] = (float)m;
+    }
+    long cReal = (nReal + Lc - 1) / Lc - c0;
+    cReal = cReal < CH ? cReal : CH;
+    if (cReal <= 0) return;
+    int cl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * g + v;
+        cl[v] = c < cReal ? c : (int)cReal - 1;
+    }
+    // initial state: zeros or the chunk's carry; first leaves
+    float xh[4], xc[4], vl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const long last = pos((c0 + cl[v] + 1) * Lc - 1);
+        xh[v] = (PASS2 && rowOk) ? X[last * na + r] : 0.0f;
+        xc[v] = 0.0f;
+        vl[v] = rowOk ? X[pos((c0 + cl[v]) * Lc) * na + r] : 0.0f;
+    }
+    for (int t = threadIdx.x; t < CH * PITCH; t += NT) xsd[t] = 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        if (rowOk) xsd[(4 * g + v) * PITCH + r] = xh[v];
+    // pass 2: the state of the chunk's first element goes out (over its leaf)
+    if (PASS2 && rowOk)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            const long e = (c0 + c) * Lc;
+            if (c < cReal && e < nReal) X[pos(e) * na + r] = xh[v];
+        }
+    __syncthreads();
+    const int steps = PASS2 ? Lc - 1 : Lc;
+    for (int j = 0; j < steps; ++j) {
+        float vn[4];
+        const bool more = j + 1 < steps || !PASS2;
+        const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            vn[v] = (rowOk && more) ? X[pos((c0 + cl[v]) * Lc + jn) * na + r] : 0.0f;
+        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4* ab = reinterpret_cast<const floatx4*>(xsd + l16 * PITCH + g * KS);
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+            const floatx4 a = ab[q];
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bm[4 * q], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bm[4 * q + 1], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bm[4 * q + 2], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bm[4 * q + 3], acc1, 0, 0, 0);
+        }
+        __syncthreads();                  // every wave has read the state
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            const float y = (acc0[v] + acc1[v] + vl[v]) - xc[v];
+            const float sum = xh[v] + y;
+            xc[v] = (sum - xh[v]) - y;
+            xh[v] = sum;
+            if (rowOk) xsd[c * PITCH + r] = sum;
+            if (PASS2 && rowOk) {
+                const long e = (c0 + c) * Lc + j + 1;
+                if (c < cReal && e < nReal) X[pos(e) * na + r] = sum - xc[v];
+            }
+            vl[v] = vn[v];
+        }
+        __syncthreads();
+    }
+    if (PASS2) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * g + v;
+        if (rowOk && c < cReal) X[pos((c0 + c + 1) * Lc - 1) * na + r] = xh[v] - xc[v];
+    }
+}
+
+bool chunks_dense_supported(int na) { return na >= 1 && na <= 256; }
+
+void chunks_dense(hipStream_t s, bool pass2, int na, int lc, long S, long nReal, long nElem,
+                  const double* P64, float* X)
+{
+    if (!chunks_dense_supported(na) || lc < 1 || S < 1 || (nElem >> lc) < 1 || nReal < 1 ||
+        nReal > nElem)
+        throw std::invalid_argument("chunks_dense: bad shape");
+    const long nChunks = nElem >> lc;
+    const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
+    auto launch = [&](auto np) {
+        constexpr int NP = decltype(np)::value;
+        const size_t lds = (size_t)kChunkCols * (NP + 4) * sizeof(float);
+        if (pass2)
+            hipLaunchKernelGGL((k_chunks_dense<NP, true>), grid, dim3(NP * 4), lds, s, na, lc, S,
+                               nReal, P64, X);
+        else
+            hipLaunchKernelGGL((k_chunks_dense<NP, false>), grid, dim3(NP * 4), lds, s, na, lc, S,
+                               nReal, P64, X);
+    };
+    if (na <= 16)
+        launch(std::integral_constant<int, 16>());
+    else if (na <= 32)
+        launch(std::integral_constant<int, 32>());
+    else if (na <= 64)
+        launch(std::integral_constant<int, 64>());
+    else if (na <= 128)
+        launch(std::integral_constant<int, 128>());
+    else
+        launch(std::integral_constant<int, 256>());
+    check_launch("chunks_dense");
+}
+
 bool abm_chunks_supported(int n) { return n >= 1 && n <= 64; }
 
 void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,

@@ -221,19 +221,31 @@ bool ScanCore::structured_level0(long nReal, long nPad) const
            chunk_log(nReal, nPad) == 0;
 }
 
-// ABM with n <= 64 and FP32 lowest levels: levels 0..5 run as recurrences over
-// chunks of 64 elements (the companion map needs only its n x 4n dense block
-// per step), when at least one tree level is launched above them.
+// The lowest levels as chunk recurrences (FP32 levels, n_a <= 256, ABM with
+// n <= 64): levels 0..lc-1 over chunks of 2^lc elements (ABM: only the
+// companion map's n x 4n dense block per step).  lc is 6, or less so that a
+// launch has at least 1024 chunks (64 work-groups); at least one tree level
+// is launched above them.
 int ScanCore::chunk_log(long nReal, long nPad) const
 {
-    constexpr int kChunkLog = 6;
-    if (method_ != kABM || xIsDouble() || !k::abm_chunks_supported(n_) || !PTm_.get() ||
-        (level_kernel() & 9) != 0)
+    constexpr int kChunkLog = 6, kMinChunksLog = 10;
+    if (xIsDouble() || (level_kernel() & 9) != 0) return 0;
+    if (method_ == kABM ? !k::abm_chunks_supported(n_) : !k::chunks_dense_supported(na_))
         return 0;
-    for (int l = 0; l < kChunkLog; ++l)
+    const int lc = std::min(kChunkLog, ilog2(nPad) - kMinChunksLog);
+    if (lc < 2 || top_level(nReal, nPad) <= lc) return 0;
+    for (int l = 0; l < lc; ++l)
         if (level_is_f64(l) || level_is_delta(l)) return 0;
-    if (ilog2(nPad) <= kChunkLog || top_level(nReal, nPad) <= kChunkLog) return 0;
-    return kChunkLog;
+    return lc;
+}
+
+void ScanCore::chunks(hipStream_t s, bool pass2, long nReal, long nPad)
+{
+    if (method_ == kABM)
+        k::abm_chunks(s, pass2, n_, chunk_log(nReal, nPad), nReal, nPad, PT64_.get(), X32_.get());
+    else
+        k::chunks_dense(s, pass2, na_, chunk_log(nReal, nPad), 1, nReal, nPad, PT64_.get(),
+                        X32_.get());
 }
 
 void ScanCore::leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs,
@@ -436,7 +448,7 @@ void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
     const int lt = top_level(nReal, nPad);
     int l = 0;
     if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
-        k::abm_chunks(s, false, n_, lc, nReal, nPad, PT64_.get(), X32_.get());
+        chunks(s, false, nReal, nPad);
         l = lc;
     } else if (lt > 0 && structured_level0(nReal, nPad)) {
         const bool s1 = structured_level1(nReal, nPad);
@@ -531,7 +543,7 @@ void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
     const int lt = top_level(nReal, nPad);
     if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
         for (int l = lt - 1; l >= lc; --l) level(s, true, l, pairs_at(nReal, l));
-        k::abm_chunks(s, true, n_, lc, nReal, nPad, PT64_.get(), X32_.get());
+        chunks(s, true, nReal, nPad);
         return;
     }
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);

@@ -92,7 +92,7 @@ public:
 
     bool structured_level0(long nReal, long nPad) const;
     bool structured_level1(long nReal, long nPad) const;
-    // log2 of the chunk length when the lowest ABM levels run as chunk
+    // log2 of the chunk length when the lowest levels run as chunk
     // recurrences (0: they run as tree levels)
     int chunk_log(long nReal, long nPad) const;
 
@@ -113,6 +113,8 @@ private:
     bool level_is_delta(int l) const;
     void level(hipStream_t s, bool down, int l, long np);
     void level_with(hipStream_t s, bool down, int l, long np, int path);
+    // the chunk recurrences of the lowest levels (pass 1 or 2)
+    void chunks(hipStream_t s, bool pass2, long nReal, long nPad);
     std::vector<int> level_candidates(int l, long np) const;
 
     int method_, n_, m_, p_, na_;
