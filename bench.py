@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the level autotuner")
+    ap.add_argument("--graph", action="store_true",
+                    help="time replays of the solve captured in a HIP graph instead of eager solves")
     ap.add_argument("--batch", type=int, default=512, help="systems in the batch legs (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=2)
     ap.add_argument("--lanes", type=int, default=8)
@@ -319,8 +321,13 @@ def main():
     # the autotuner picks each level's kernel for this shape (outside the timed region)
     tuned = 0 if args.no_tune else plan.autotune(sp)
     torch.cuda.synchronize()
+    # the solve is recorded once into a HIP graph (no memset nodes) and replayed
+    if args.graph:
+        plan.capture(*ptrs, sp)
+        torch.cuda.synchronize()
+    run = (lambda: plan.replay(sp)) if args.graph else (lambda: plan.execute(*ptrs, sp))
     for _ in range(args.warmup):
-        plan.execute(*ptrs, sp)
+        run()
     torch.cuda.synchronize()
 
     # timed region: K solves back to back on one stream
@@ -328,7 +335,7 @@ def main():
     torch.cuda.synchronize()
     ev0.record(stream)
     for _ in range(args.steps):
-        plan.execute(*ptrs, sp)
+        run()
     ev1.record(stream)
     torch.cuda.synchronize()
     ms_total = ev0.elapsed_time(ev1)
@@ -383,6 +390,7 @@ def main():
                          (n_scan * 8 * s.n + (n_scan >> lc) * 4 * na) if lc else None},
         "phases_ms": ph,
         "autotuned_level_launches": tuned,
+        "launch": "HIP graph replay" if args.graph else "eager",
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "plan_bytes": plan.footprint,
     }

@@ -2405,13 +2405,30 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
             double v[8];
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) v[jj] = 0.0;
-            if (useC)
-                for (int c = 0; c < n; ++c) {
+            if (useC) {
+                // two rows of C^T per batch, so their scalar loads are in
+                // flight together
+                int c = 0;
+                for (; c + 2 <= n; c += 2) {
+                    double cr[2][8], xc[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        xc[q] = (double)x[c + q];
+#pragma unroll
+                        for (int jj = 0; jj < 8; ++jj) cr[q][jj] = CT[(long)(c + q) * pp + jb + jj];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int jj = 0; jj < 8; ++jj) v[jj] = fma(cr[q][jj], xc[q], v[jj]);
+                }
+                for (; c < n; ++c) {
                     const double xc = (double)x[c];
                     const double* cr = CT + (long)c * pp + jb;
 #pragma unroll
                     for (int jj = 0; jj < 8; ++jj) v[jj] = fma(cr[jj], xc, v[jj]);
                 }
+            }
             if (useD)
                 for (int c0 = 0; c0 < m; c0 += 8) {
                     double uv[8];   // eight inputs in flight (indices past m repeat m - 1)
