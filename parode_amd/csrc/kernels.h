@@ -116,6 +116,13 @@ inline bool level_mfma_splits(int na) { return na > 256; }
 // the state the down-sweep leaves there and writes the last block of every
 // real element's state, X[e][3n:4n), for the outputs.  Reads the leaves'
 // rows [3n, 4n).
+// The same ABM chunk recurrences for an FP64 map, with the companion block
+// row factored as x3 + A w1 + A^2 w2 (w1, w2: combinations of the state's four
+// blocks with the ABM coefficients of step h): 2n products per row per step.
+// A64 and A264 are A and A^2 in FP64, row-major n x n.
+void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
+                         double h, const double* A64, const double* A264, float* X);
+
 // Chunk recurrences of a dense map P (FP64 transposed, P64[k][i] = P[i][k],
 // na x na, na <= 256) over elements at X rows e S + S - 1, e < nElem (nReal
 // of them real), chunks of 2^lc elements: pass 1 writes each chunk's aggregate

@@ -1682,6 +1682,184 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks(int n, int lc, long nReal
     }
 }
 
+// ABM chunk recurrences with an FP64 map, factored: the companion map's last
+// block row is M[3n:4n, :] = [b0 A^2, a1 A + b1 A^2, a2 A + b2 A^2,
+// I + a3 A + b3 A^2] (AdamsMoulton.cpp:258-269), so a step is
+//   x_new = x3 + A w1 + A^2 w2 + v_e,  w1 = sum_b a_b x_b,  w2 = sum_b b_b x_b
+// (x_b the state's blocks, oldest first; a_0 = 0): 2n products per row
+// instead of 4n.  The lane that forms entry (chunk, row) of the new block
+// keeps that entry's last four values in registers (the shift of the
+// companion state), forms its w1 and w2 there and writes them to LDS, double
+// buffered, so one barrier per step suffices; the newest value is a
+// compensated running sum as in k_abm_chunks.  Tiles, lane layouts and the
+// passes are those of k_abm_chunks, with A = [w1 w2] (16 chunks x 2 NP) and
+// B = [A A^2]^T rows of the wave's tile, formed from the FP64 products and
+// rounded once.
+struct AbmFactors {
+    float a[4], b[4];
+};
+
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long nReal,
+                                                           const double* __restrict__ A64,
+                                                           const double* __restrict__ A264,
+                                                           AbmFactors f, float* __restrict__ X)
+{
+#pragma clang fp contract(off)
+    constexpr int CH = kChunkCols;
+    constexpr int KS = NP / 2;            // k-steps of four over the 2 NP padded [w1 w2]
+    constexpr int PITCH = 2 * NP + 4;     // 4 mod 64 words for NP = 64; 16-byte aligned rows
+    constexpr int NT = NP * 4;
+    extern __shared__ float wbuf[];       // [2][CH][PITCH]
+    const int na = 4 * n, Lc = 1 << lc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int r = wave * 16 + l16;
+    const bool rowOk = r < n;
+    const long c0 = (long)blockIdx.x * CH;
+    // B fragments: lane group g takes the padded states [g KS, (g+1) KS) of
+    // [w1 w2]; padded state b NP + rr (b = 0: A, 1: A^2) is real when rr < n
+    float bm[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int kp = g * KS + ks, b = kp / NP, rr = kp % NP;
+        double m = 0.0;
+        if (rowOk && rr < n) m = (b == 0 ? A64 : A264)[(long)r * n + rr];
+        bm[ks] = (float)m;
+    }
+    long cReal = (nReal + Lc - 1) / Lc - c0;
+    cReal = cReal < CH ? cReal : CH;
+    if (cReal <= 0) return;
+    int cl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * g + v;
+        cl[v] = c < cReal ? c : (int)cReal - 1;
+    }
+    // this lane's entries of the state's four blocks, oldest first, and the
+    // compensation of the newest
+    float hs[4][4], xc[4], xh[4], vl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const long last = ((c0 + cl[v] + 1) * Lc - 1) * na;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) hs[v][b] = (PASS2 && rowOk) ? X[last + b * n + r] : 0.0f;
+        xh[v] = hs[v][3];
+        xc[v] = 0.0f;
+        vl[v] = rowOk ? X[((c0 + cl[v]) * Lc) * na + 3 * n + r] : 0.0f;
+    }
+    auto put_w = [&](int buf) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            float w1 = 0.0f, w2 = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                w1 = w1 + f.a[b] * hs[v][b];
+                w2 = w2 + f.b[b] * hs[v][b];
+            }
+            if (r < NP) {
+                wbuf[(buf * CH + c) * PITCH + r] = rowOk ? w1 : 0.0f;
+                wbuf[(buf * CH + c) * PITCH + NP + r] = rowOk ? w2 : 0.0f;
+            }
+        }
+    };
+    put_w(0);
+    __syncthreads();   // w of the first step is in LDS, the first leaves in registers
+    // pass 2: the state of the chunk's first element goes out (over its leaf)
+    if (PASS2 && rowOk)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            const long e = (c0 + c) * Lc;
+            if (c < cReal && e < nReal) X[e * na + 3 * n + r] = hs[v][3];
+        }
+    const int steps = PASS2 ? Lc - 1 : Lc;
+    int cur = 0;
+    for (int j = 0; j < steps; ++j) {
+        float vn[4];
+        const bool more = j + 1 < steps || !PASS2;
+        const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            vn[v] = (rowOk && more) ? X[((c0 + cl[v]) * Lc + jn) * na + 3 * n + r] : 0.0f;
+        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4* ab =
+            reinterpret_cast<const floatx4*>(wbuf + (cur * CH + l16) * PITCH + g * KS);
+        floatx4 a[KS / 4];
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) a[q] = ab[q];
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][0], bm[4 * q], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][1], bm[4 * q + 1], acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][2], bm[4 * q + 2], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q][3], bm[4 * q + 3], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            // Kahan: x3 + (A w1 + A^2 w2 + leaf), the lost low part in xc
+            const float y = (acc0[v] + acc1[v] + vl[v]) - xc[v];
+            const float sum = xh[v] + y;
+            xc[v] = (sum - xh[v]) - y;
+            xh[v] = sum;
+            const float xv = sum - xc[v];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) hs[v][b] = hs[v][b + 1];
+            hs[v][3] = xv;
+            if (PASS2 && rowOk) {
+                const long e = (c0 + c) * Lc + j + 1;
+                if (c < cReal && e < nReal) X[e * na + 3 * n + r] = xv;
+            }
+            vl[v] = vn[v];
+        }
+        put_w(cur ^ 1);                   // the other buffer: no wave reads it this step
+        cur ^= 1;
+        __syncthreads();
+    }
+    if (PASS2) return;
+    // pass 1: the aggregate (the four blocks) to the chunk's last element
+    if (rowOk)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = 4 * g + v;
+            if (c < cReal)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) X[((c0 + c + 1) * Lc - 1) * na + b * n + r] = hs[v][b];
+        }
+}
+
+void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
+                         double h, const double* A64, const double* A264, float* X)
+{
+    if (!abm_chunks_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
+        throw std::invalid_argument("abm_chunks_factored: bad shape");
+    const AbmCoef c = abm_coef(h);
+    // blocks oldest first: x_{k-3} (A^2 only), x_{k-2}, x_{k-1}, x_k (+ I)
+    const AbmFactors f = {{0.0f, (float)c.a2A, (float)c.a1A, (float)c.a0A},
+                          {(float)c.a3A2, (float)c.a2A2, (float)c.a1A2, (float)c.a0A2}};
+    const long nChunks = nPad >> lc;
+    const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
+    auto launch = [&](auto np) {
+        constexpr int NP = decltype(np)::value;
+        const size_t lds = (size_t)2 * kChunkCols * (2 * NP + 4) * sizeof(float);
+        if (pass2)
+            hipLaunchKernelGGL((k_abm_chunks_fac<NP, true>), grid, dim3(NP * 4), lds, s, n, lc,
+                               nReal, A64, A264, f, X);
+        else
+            hipLaunchKernelGGL((k_abm_chunks_fac<NP, false>), grid, dim3(NP * 4), lds, s, n, lc,
+                               nReal, A64, A264, f, X);
+    };
+    if (n <= 16)
+        launch(std::integral_constant<int, 16>());
+    else if (n <= 32)
+        launch(std::integral_constant<int, 32>());
+    else
+        launch(std::integral_constant<int, 64>());
+    check_launch("abm_chunks_factored");
+}
+
 // Chunk recurrences of a dense map (RK4's M, or the ABM power P_l over chunk
 // aggregates): x_new = x + (P - I) x + v_e over the whole state, for elements
 // at X rows pos(e) = e S + S - 1 (S = 1: every element; S = 2^l: the values

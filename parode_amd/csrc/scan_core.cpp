@@ -120,6 +120,7 @@ bool ScanCore::level_is_delta(int l) const
 // ------------------------------------------------------------------ step map
 void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double h)
 {
+    h_ = h;
     const int n = n_, m = m_;
     const size_t nn = (size_t)n * n, nm = (size_t)n * (m > 0 ? m : 1);
     double* MT = PT64_.get();
@@ -241,7 +242,12 @@ int ScanCore::chunk_log(long nReal, long nPad) const
 
 void ScanCore::chunks(hipStream_t s, bool pass2, long nReal, long nPad)
 {
-    if (method_ == kABM)
+    // an FP64 map: the companion block row factored through A and A^2 (the
+    // plain path keeps the reference's float M)
+    if (method_ == kABM && opt_.precision != kPlain32)
+        k::abm_chunks_factored(s, pass2, n_, chunk_log(nReal, nPad), nReal, nPad, h_, A64_.get(),
+                               prod64_.get(), X32_.get());
+    else if (method_ == kABM)
         k::abm_chunks(s, pass2, n_, chunk_log(nReal, nPad), nReal, nPad, PT64_.get(), X32_.get());
     else
         k::chunks_dense(s, pass2, na_, chunk_log(nReal, nPad), 1, nReal, nPad, PT64_.get(),

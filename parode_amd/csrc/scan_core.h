@@ -131,6 +131,7 @@ private:
     DevArray<double> BfT64_;
     DevArray<double> CT64_, DT64_;   // C^T and D^T in FP64 for the outputs
     double leafG_[5][2] = {};
+    double h_ = 0.0;   // the step of the map built last
     bool leafFac_ = false;
     DevArray<float> Bbar32_, BbT32_;
     DevArray<float> CT_, DT_;        // C^T and D^T for the output stage
