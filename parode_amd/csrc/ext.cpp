@@ -22,6 +22,8 @@ struct Plan {
     int device;
     std::unique_ptr<parode::ScanCore> core;
     hipEvent_t ev[8] = {};
+    // the power table and the ABM start-up run here, beside the leaves
+    parode::ScanCore::SideStream side;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void drop_graph()
@@ -36,6 +38,18 @@ struct Plan {
         drop_graph();
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (side.fork) (void)hipEventDestroy(side.fork);
+        if (side.join) (void)hipEventDestroy(side.join);
+        if (side.stream) (void)hipStreamDestroy(side.stream);
+    }
+    const parode::ScanCore::SideStream* side_stream()
+    {
+        if (!side.stream) {
+            PARODE_HIP(hipStreamCreate(&side.stream));
+            PARODE_HIP(hipEventCreateWithFlags(&side.fork, hipEventDisableTiming));
+            PARODE_HIP(hipEventCreateWithFlags(&side.join, hipEventDisableTiming));
+        }
+        return &side;
     }
 };
 
@@ -47,7 +61,7 @@ void run_plan(Plan* pl, const float* dA, const float* dB, const float* dC, const
               const float* dx0, const void* dSamples, float* dY, hipStream_t s, bool mark)
 {
     pl->core->solve(s, pl->points, pl->h, dA, dB, dC, dD, dx0, dSamples, pl->uDouble, dY,
-                    mark ? pl->ev : nullptr);
+                    mark ? pl->ev : nullptr, pl->side_stream());
 }
 
 int shape_error(int solver, int n, int m, long points)

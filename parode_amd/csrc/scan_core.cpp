@@ -449,12 +449,15 @@ static k::LevelRows abm_rows(int n, int level, bool down, bool next)
     return r;
 }
 
-void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad)
+void ScanCore::sweep_up(hipStream_t s, long nReal, long nPad, hipEvent_t waitTables)
 {
     const int lt = top_level(nReal, nPad);
     int l = 0;
-    if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
-        chunks(s, false, nReal, nPad);
+    const int lc = chunk_log(nReal, nPad);
+    // the chunk recurrences read only the map; tree levels read the tables
+    if (lc) chunks(s, false, nReal, nPad);
+    if (waitTables) PARODE_HIP(hipStreamWaitEvent(s, waitTables, 0));
+    if (lc) {   // levels below lc: chunk recurrences
         l = lc;
     } else if (lt > 0 && structured_level0(nReal, nPad)) {
         const bool s1 = structured_level1(nReal, nPad);
@@ -646,30 +649,42 @@ void ScanCore::abm_startup(hipStream_t s, const float* dA, const float* dB, cons
 // ---------------------------------------------------------------- one solve
 void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, const float* dB,
                      const float* dC, const float* dD, const float* dx0, const void* dU,
-                     bool uDouble, float* dY, hipEvent_t* marks)
+                     bool uDouble, float* dY, hipEvent_t* marks, const SideStream* side)
 {
     const long nScan = method_ == kRK4 ? points : points - 3;
     const long nPad = next_pow2(nScan);
     if (nScan < 1 || nPad > nPadMax_) throw std::invalid_argument("ScanCore::solve: bad length");
+    if (side && (!side->stream || !side->fork || !side->join))
+        throw std::invalid_argument("ScanCore::solve: incomplete side stream");
     auto tick = [&](int i) {
         if (marks) PARODE_HIP(hipEventRecord(marks[i], s));
     };
     tick(0);
     build_map(s, dA, dB, h);
-    build_powers(s);
-    tick(1);
+    // the power table and the ABM start-up: on the side stream when given
+    hipStream_t ps = s;
+    if (side) {
+        PARODE_HIP(hipEventRecord(side->fork, s));
+        PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+        ps = side->stream;
+    }
+    build_powers(ps);
     float* yScan = dY;
     if (method_ == kRK4) {
-        k::to_f64<float>(s, n_, dx0, seed());
-        leaves(s, dU, uDouble, 2, 0, 1, nScan, nPad);
+        k::to_f64<float>(ps, n_, dx0, seed());
     } else {
-        abm_startup(s, dA, dB, dx0, dU, uDouble, h, seed());
-        startup_outputs(s, dC, dD, seed(), dU, uDouble, dY);
+        abm_startup(ps, dA, dB, dx0, dU, uDouble, h, seed());
+        startup_outputs(ps, dC, dD, seed(), dU, uDouble, dY);
         yScan = dY + 3L * nOut();
-        leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
     }
+    if (side) PARODE_HIP(hipEventRecord(side->join, ps));
+    tick(1);
+    if (method_ == kRK4)
+        leaves(s, dU, uDouble, 2, 0, 1, nScan, nPad);
+    else
+        leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
     tick(2);
-    sweep_up(s, nScan, nPad);
+    sweep_up(s, nScan, nPad, side ? side->join : nullptr);
     tick(3);
     sweep_top(s, nScan, nPad, seed(), tot());
     tick(4);

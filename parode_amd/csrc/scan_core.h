@@ -61,7 +61,9 @@ public:
     // top_level(), one work-group for the top of the tree (its up levels, the
     // root with the seed, its down levels), down-sweep launches below.
     int top_level(long nReal, long nPad) const;
-    void sweep_up(hipStream_t s, long nReal, long nPad);
+    // waitTables: an event s waits on before the first launch that reads the
+    // level tables (after the first chunk pass, if any)
+    void sweep_up(hipStream_t s, long nReal, long nPad, hipEvent_t waitTables = nullptr);
     void sweep_top(hipStream_t s, long nReal, long nPad, const double* dSeed, double* dTot);
     void sweep_down(hipStream_t s, long nReal, long nPad);
     // seedOut = M^nPad * seed + tot (the accumulator carried to the next batch)
@@ -80,9 +82,16 @@ public:
     // nPadMax).  Samples on the h/2 grid, (2*points+1) rows of m.  marks, when
     // given, are 7 events recorded before each of the six phases and after the
     // last.  Asynchronous on s.
+    // With side, the power table and the ABM start-up run on side.stream
+    // (forked from s after the map, joined before the first tree level), in
+    // parallel with the leaves and the first chunk pass.
+    struct SideStream {
+        hipStream_t stream = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+    };
     void solve(hipStream_t s, long points, double h, const float* dA, const float* dB,
                const float* dC, const float* dD, const float* dx0, const void* dU, bool uDouble,
-               float* dY, hipEvent_t* marks = nullptr);
+               float* dY, hipEvent_t* marks = nullptr, const SideStream* side = nullptr);
 
     double* seed() const { return seed_.get(); }
     double* tot() const { return tot_.get(); }
