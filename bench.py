@@ -31,7 +31,7 @@ PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 TRAFFIC_CHUNK_PASS2 = (137808 * 2.0 + 262737) * 1024
 
 
-def level_flops(na, n_scan, n_leaf=None, chunk_log=0):
+def level_flops(na, n_scan, n_leaf=None, chunk_log=0, factored=False):
     """Algorithmic flops of the scan (both sweeps): 2 na^2 per pair, pairs of
     level l = ceil(N / 2^(l+1)).  With n_leaf (ABM, n = na / 4: the companion
     map M has rows below 3n that are shifts by n, M^2 rows below 2n that are
@@ -41,15 +41,18 @@ def level_flops(na, n_scan, n_leaf=None, chunk_log=0):
                down 2 n na (only the last n rows, which the outputs read)
       level 1: up 2 (2n)(2n), down 2 (2n) na.
     With chunk_log = lc (chunk recurrences), levels below lc are replaced by
-    two passes per element (ABM: 2 n na, the n x 4n dense block of M per step;
-    RK4: 2 na^2), and only levels >= lc run as tree levels.
+    two passes per element (ABM: 2 n na, the n x 4n dense block of M per step,
+    or 2 n (2n) with the FP64 map factored as x3 + A w1 + A^2 w2; RK4: 2 na^2),
+    and only levels >= lc run as tree levels.
     Returns (flops, tree pairs)."""
     L = max(0, (n_scan - 1).bit_length())
     per = [(n_scan + (2 << l) - 1) // (2 << l) for l in range(L)]
     if chunk_log:
         # levels below chunk_log (ABM: the n x 4n block per step; RK4: the
         # dense map), two passes over every element
-        per_step = 2.0 * n_leaf * na if n_leaf else 2.0 * na * na
+        # (an FP64 ABM map factored through A and A^2: 2 n x n products)
+        per_step = (2.0 * n_leaf * (2 * n_leaf) if factored else 2.0 * n_leaf * na) \
+            if n_leaf else 2.0 * na * na
         flops = 2 * per_step * n_scan
         tree = per[chunk_log:]
         return flops + 2 * 2.0 * na * na * sum(tree), sum(tree)
@@ -353,7 +356,8 @@ def main():
     na = s.n_scan
     n_scan = s.steps if s.method == 0 else s.steps - 3
     lc = plan.chunk_log
-    flops, pairs = level_flops(na, n_scan, s.n if s.method == 1 else None, lc)
+    flops, pairs = level_flops(na, n_scan, s.n if s.method == 1 else None, lc,
+                               factored=prec == api.MIXED)
     sweep_ms = ph["up_sweep"] + ph["down_sweep"]
     achieved = flops / (sweep_ms * 1e-3)
     peak = PEAK["fp64"] if prec == api.FP64 else PEAK["fp32"]
@@ -378,7 +382,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": TRAFFIC_CHUNK_PASS2 if (args.config == 1 and not args.points
-                                                        and lc == 6) else None,
+                                                        and lc == 7) else None,
                      "traffic_launch": "second chunk pass (k_abm_chunks)" if lc else None,
                      "kernel": ("k_abm_chunks (both passes) and the tree levels above them"
                                 if lc else "k_level_mfma (every level launch of both sweeps)"),

@@ -118,7 +118,8 @@ bool ScanCore::level_is_delta(int l) const
 }
 
 // ------------------------------------------------------------------ step map
-void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double h)
+void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double h,
+                         const SideStream* side)
 {
     h_ = h;
     const int n = n_, m = m_;
@@ -172,7 +173,6 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
                            Bbar32_.get());
         } else {
             if (m > 0) k::gemm_f64(s, n, n, m, A, B, AB, mm);
-            k::abm_combine(s, false, n, m, h, A, A2, B, AB, MT, Bbar64_.get(), Bbar32_.get());
             if (m > 0) {   // leaves as [B | AB] times two combinations of the samples
                 k::transpose<double>(s, n, m, B, BfT64_.get());
                 k::transpose<double>(s, n, m, AB, BfT64_.get() + (size_t)m * n);
@@ -182,7 +182,21 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
                     for (int g = 0; g < 2; ++g) leafG_[q][g] = G.g[q][g];
                 leafFac_ = true;
             }
+            // what the factored leaves and the chunk recurrences read is
+            // ready: the companion map and B-bar (for the power table, and the
+            // unfactored leaves, which large input counts would take) on the
+            // side stream
+            if (side && m > 0 && m <= 16) {
+                PARODE_HIP(hipEventRecord(side->fork, s));
+                PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+                s = side->stream;
+            }
+            k::abm_combine(s, false, n, m, h, A, A2, B, AB, MT, Bbar64_.get(), Bbar32_.get());
         }
+    }
+    if (side && s != side->stream) {
+        PARODE_HIP(hipEventRecord(side->fork, s));
+        PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
     }
     const int kq = method_ == kABM ? 5 : 3;
     if (m > 0) {
@@ -224,12 +238,12 @@ bool ScanCore::structured_level0(long nReal, long nPad) const
 
 // The lowest levels as chunk recurrences (FP32 levels, n_a <= 256, ABM with
 // n <= 64): levels 0..lc-1 over chunks of 2^lc elements (ABM: only the
-// companion map's n x 4n dense block per step).  lc is 6, or less so that a
+// companion map's dense block row per step).  lc is 7, or less so that a
 // launch has at least 1024 chunks (64 work-groups); at least one tree level
 // is launched above them.
 int ScanCore::chunk_log(long nReal, long nPad) const
 {
-    constexpr int kChunkLog = 6, kMinChunksLog = 10;
+    constexpr int kChunkLog = 7, kMinChunksLog = 10;
     if (xIsDouble() || (level_kernel() & 9) != 0) return 0;
     if (method_ == kABM ? !k::abm_chunks_supported(n_) : !k::chunks_dense_supported(na_))
         return 0;
@@ -660,14 +674,10 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
         if (marks) PARODE_HIP(hipEventRecord(marks[i], s));
     };
     tick(0);
-    build_map(s, dA, dB, h);
-    // the power table and the ABM start-up: on the side stream when given
-    hipStream_t ps = s;
-    if (side) {
-        PARODE_HIP(hipEventRecord(side->fork, s));
-        PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
-        ps = side->stream;
-    }
+    // the power table and the ABM start-up (and for an FP64 ABM map, the
+    // map's last launches): on the side stream when given
+    build_map(s, dA, dB, h, side);
+    hipStream_t ps = side ? side->stream : s;
     build_powers(ps);
     float* yScan = dY;
     if (method_ == kRK4) {

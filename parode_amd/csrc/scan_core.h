@@ -53,7 +53,14 @@ public:
     static size_t estimate_footprint(int method, int n, int m, long nPadMax, ScanOptions opt,
                                      int p = 0);
 
-    void build_map(hipStream_t s, const float* dA, const float* dB, double h);
+    // With side, the launches that the leaves and the chunk recurrences do not
+    // need are forked onto side->stream (fork event recorded on s).
+    struct SideStream {
+        hipStream_t stream = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+    };
+    void build_map(hipStream_t s, const float* dA, const float* dB, double h,
+                   const SideStream* side = nullptr);
     void build_powers(hipStream_t s);
     void leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs, long nReal,
                 long nPad);
@@ -85,10 +92,6 @@ public:
     // With side, the power table and the ABM start-up run on side.stream
     // (forked from s after the map, joined before the first tree level), in
     // parallel with the leaves and the first chunk pass.
-    struct SideStream {
-        hipStream_t stream = nullptr;
-        hipEvent_t fork = nullptr, join = nullptr;
-    };
     void solve(hipStream_t s, long points, double h, const float* dA, const float* dB,
                const float* dC, const float* dD, const float* dx0, const void* dU, bool uDouble,
                float* dY, hipEvent_t* marks = nullptr, const SideStream* side = nullptr);
