@@ -25,10 +25,10 @@ sys.path.insert(0, ROOT)
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "hbm": 8.0e12}
 
 # HBM bytes of the largest launch of the scan at the default workload, the
-# second chunk pass (k_abm_chunks, 2^20 elements in chunks of 64), from
+# second chunk pass (k_abm_chunks_fac, 2^20 elements in chunks of 128), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE counters, FETCH scaled by 2.0 (what the
-# counter reads for a plain copy of known size): (137808 * 2.0 + 262737) KiB.
-TRAFFIC_CHUNK_PASS2 = (137808 * 2.0 + 262737) * 1024
+# counter reads for a plain copy of known size): (134471 * 2.0 + 262364) KiB.
+TRAFFIC_CHUNK_PASS2 = (134471 * 2.0 + 262364) * 1024
 
 
 def level_flops(na, n_scan, n_leaf=None, chunk_log=0, factored=False):

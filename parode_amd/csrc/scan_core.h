@@ -128,6 +128,7 @@ private:
     // the chunk recurrences of the lowest levels (pass 1 or 2)
     void chunks(hipStream_t s, bool pass2, long nReal, long nPad);
     std::vector<int> level_candidates(int l, long np) const;
+    void zero_trajectory(hipStream_t s, long nPad);
 
     int method_, n_, m_, p_, na_;
     bool zC_, zD_;
