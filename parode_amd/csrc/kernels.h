@@ -120,6 +120,7 @@ inline bool level_mfma_splits(int na) { return na > 256; }
 // row factored as x3 + A w1 + A^2 w2 (w1, w2: combinations of the state's four
 // blocks with the ABM coefficients of step h): 2n products per row per step.
 // A64 and A264 are A and A^2 in FP64, row-major n x n.
+bool abm_chunks_factored_supported(int n);   // n <= 128
 void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                          double h, const double* A64, const double* A264, float* X);
 

@@ -1830,10 +1830,13 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
         }
 }
 
+bool abm_chunks_factored_supported(int n) { return n >= 1 && n <= 128; }
+
 void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                          double h, const double* A64, const double* A264, float* X)
 {
-    if (!abm_chunks_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
+    if (!abm_chunks_factored_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 ||
+        nReal > nPad)
         throw std::invalid_argument("abm_chunks_factored: bad shape");
     const AbmCoef c = abm_coef(h);
     // blocks oldest first: x_{k-3} (A^2 only), x_{k-2}, x_{k-1}, x_k (+ I)
@@ -1855,8 +1858,10 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
         launch(std::integral_constant<int, 16>());
     else if (n <= 32)
         launch(std::integral_constant<int, 32>());
-    else
+    else if (n <= 64)
         launch(std::integral_constant<int, 64>());
+    else
+        launch(std::integral_constant<int, 128>());
     check_launch("abm_chunks_factored");
 }
 

@@ -236,8 +236,9 @@ bool ScanCore::structured_level0(long nReal, long nPad) const
            chunk_log(nReal, nPad) == 0;
 }
 
-// The lowest levels as chunk recurrences (FP32 levels, n_a <= 256, ABM with
-// n <= 64): levels 0..lc-1 over chunks of 2^lc elements (ABM: only the
+// The lowest levels as chunk recurrences (FP32 levels; RK4 n_a <= 256, ABM
+// n <= 64, or n <= 128 with an FP64 map): levels 0..lc-1 over chunks of 2^lc
+// elements (ABM: only the
 // companion map's dense block row per step).  lc is 7, or less so that a
 // launch has at least 1024 chunks (64 work-groups); at least one tree level
 // is launched above them.
@@ -245,8 +246,10 @@ int ScanCore::chunk_log(long nReal, long nPad) const
 {
     constexpr int kChunkLog = 7, kMinChunksLog = 10;
     if (xIsDouble() || (level_kernel() & 9) != 0) return 0;
-    if (method_ == kABM ? !k::abm_chunks_supported(n_) : !k::chunks_dense_supported(na_))
-        return 0;
+    // ABM: n <= 64 with the reference's float map, n <= 128 factored (FP64 map)
+    const bool abmOk = opt_.precision == kPlain32 ? k::abm_chunks_supported(n_)
+                                                  : k::abm_chunks_factored_supported(n_);
+    if (method_ == kABM ? !abmOk : !k::chunks_dense_supported(na_)) return 0;
     const int lc = std::min(kChunkLog, ilog2(nPad) - kMinChunksLog);
     if (lc < 2 || top_level(nReal, nPad) <= lc) return 0;
     for (int l = 0; l < lc; ++l)

@@ -275,7 +275,8 @@ def test_precision_modes_against_fp64(solver, mode, bound):
 @pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-5)])
 @pytest.mark.parametrize("solver,n,m,steps,f32", [(1, 17, 1, 1000, False), (1, 40, 3, 3001, True),
                                                   (0, 5, 7, 999, False), (0, 70, 5, 2049, True),
-                                                  (1, 64, 8, 4097, False)])
+                                                  (1, 64, 8, 4097, False),
+                                                  (1, 100, 4, 5000, False)])
 def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
     """The FP64 leaves run on the matrix cores in 16 x 16 tiles: ragged row
     tiles (n not a multiple of 16), input counts that do not fill a 4-column
