@@ -120,7 +120,7 @@ inline bool level_mfma_splits(int na) { return na > 256; }
 // row factored as x3 + A w1 + A^2 w2 (w1, w2: combinations of the state's four
 // blocks with the ABM coefficients of step h): 2n products per row per step.
 // A64 and A264 are A and A^2 in FP64, row-major n x n.
-bool abm_chunks_factored_supported(int n);   // n <= 128
+bool abm_chunks_factored_supported(int n);   // n <= 256
 void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                          double h, const double* A64, const double* A264, float* X);
 
@@ -141,9 +141,10 @@ void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
 // FP64 MFMA when TT = double, with PT the transposed FP64 power, ldp = na).
 // The down-sweep first copies the pairs' X[R] into S (np * na values).
 constexpr long kKsMaxPairs = 4096;
+// delta: the table holds P - I (FP32 tables of mixed precision's delta levels)
 template <typename TT, typename TX>
 void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
-              TX* S);
+              TX* S, bool delta = false);
 
 // Levels [l0, L) of both sweeps and the root in one work-group (levels >= l64
 // from the FP64 table, the others from the FP32 one).

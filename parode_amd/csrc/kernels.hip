@@ -1256,9 +1256,11 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
     constexpr bool fromS = DOWN && SPLIT;
     const int ncol = (np - k0) < (long)TC ? (int)(np - k0) : TC;
     const bool full = ncol == TC;
-    // Up-sweep: the accumulators start from the addend X[R] (plus X[L] in the
-    // delta form), loaded before the first K-slice so their latency hides
-    // behind it, and the epilogue only stores.  Down-sweep: they start at zero
+    // Up-sweep: the accumulators start from the addend X[R], loaded before
+    // the first K-slice so their latency hides behind it, and the epilogue
+    // only stores; in the delta form they start at zero, so the small
+    // products (P - I) X[L] are summed apart, and the epilogue adds
+    // X[R] + X[L].  Down-sweep: they start at zero
     // and the epilogue loads X[R] and X[L] (a preload would raise the register
     // count past two waves per SIMD).  Columns past the tile are clamped
     // (loaded, never stored); level-0 leaf rows below rbeg are zero (one
@@ -1272,15 +1274,13 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
             for (int v = 0; v < 16; ++v) acc[ct][it][v] = 0.0f;
             const int i = ib0 + (wi + it * WGI) * 32 + l32;
             const int ic = i < na ? i : na - 1;
-            if (!DOWN && i >= rbeg) {
+            if (!DOWN && !DELTA && i >= rbeg) {
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
                     const int crc = cr < ncol ? cr : ncol - 1;
                     const long R = (k0 + crc + 1) * 2 * d - 1;
-                    float a = X[R * na + ic];
-                    if (DELTA) a += X[(R - d) * na + ic];
-                    acc[ct][it][v] = a;
+                    acc[ct][it][v] = X[R * na + ic];
                 }
             }
         }
@@ -1386,7 +1386,10 @@ __global__ __launch_bounds__(256) void k_level_mfma(int na, int nap, int nrp, lo
                 for (int v = 0; v < 16; ++v) {
                     const int cr = (wc + ct * WGC) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
                     if (!full && cr >= ncol) continue;
-                    X[((k0 + cr + 1) * 2 * d - 1) * na + i] = acc[ct][it][v];
+                    const long R = (k0 + cr + 1) * 2 * d - 1;
+                    float y = acc[ct][it][v];
+                    if (DELTA && i >= rbeg) y = X[R * na + i] + (X[(R - d) * na + i] + y);
+                    X[R * na + i] = y;
                 }
                 continue;
             }
@@ -1709,7 +1712,6 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
     constexpr int CH = kChunkCols;
     constexpr int KS = NP / 2;            // k-steps of four over the 2 NP padded [w1 w2]
     constexpr int PITCH = 2 * NP + 4;     // 4 mod 64 words for NP = 64; 16-byte aligned rows
-    constexpr int NT = NP * 4;
     extern __shared__ float wbuf[];       // [2][CH][PITCH]
     const int na = 4 * n, Lc = 1 << lc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1830,7 +1832,169 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
         }
 }
 
-bool abm_chunks_factored_supported(int n) { return n >= 1 && n <= 128; }
+// The factored ABM step for 128 < n <= 256, as two dependent products:
+//   x_new = x3 + A u + v_e,  u = w1 + A w2
+// so a CU holds only A (n x n FP32, in registers: a wave's 32 rows of A over
+// the whole padded state, split among its four lane groups) instead of
+// [A A^2], which would fill the register file.  A work-group of NP / 32 waves
+// takes 16 chunks; each wave forms two 16 x 16 tiles (rows wave*32 + 16 t),
+// so one read of a 16 x 4 operand fragment from LDS feeds both.  Per step:
+// w2 (written at the end of the previous step) -> A w2 -> u to LDS; barrier;
+// A u -> compensated update, history shift, the next w1 (registers) and w2
+// (LDS); barrier.  w2 and u have buffers of their own, so two barriers
+// per step suffice.  Lane layouts and passes are those of k_abm_chunks_fac.
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 2) void k_abm_chunks_two(int n, int lc, long nReal,
+                                                           const double* __restrict__ A64,
+                                                           AbmFactors f, float* __restrict__ X)
+{
+#pragma clang fp contract(off)
+    constexpr int CH = kChunkCols;
+    constexpr int KS = NP / 4;            // padded states per lane group
+    constexpr int PITCH = NP + 4;         // 16-byte aligned rows
+    static_assert(NP % 32 == 0 && KS % 4 == 0, "NP: a multiple of 32");
+    extern __shared__ float sbuf[];       // w2: [CH][PITCH], then u: [CH][PITCH]
+    float* const w2buf = sbuf;
+    float* const ubuf = sbuf + CH * PITCH;
+    const int na = 4 * n, Lc = 1 << lc;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    int rr[2];
+    bool ok[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        rr[t] = wave * 32 + 16 * t + l16;
+        ok[t] = rr[t] < n;
+    }
+    const long c0 = (long)blockIdx.x * CH;
+    float bm[2][KS];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = g * KS + ks;
+            bm[t][ks] = (ok[t] && k < n) ? (float)A64[(long)rr[t] * n + k] : 0.0f;
+        }
+    long cReal = (nReal + Lc - 1) / Lc - c0;
+    cReal = cReal < CH ? cReal : CH;
+    if (cReal <= 0) return;
+    int cl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = 4 * g + v;
+        cl[v] = c < cReal ? c : (int)cReal - 1;
+    }
+    float hs[2][4][4], xc[2][4], xh[2][4], vl[2][4], w1[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long last = ((c0 + cl[v] + 1) * Lc - 1) * na;
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                hs[t][v][b] = (PASS2 && ok[t]) ? X[last + b * n + rr[t]] : 0.0f;
+            xh[t][v] = hs[t][v][3];
+            xc[t][v] = 0.0f;
+            vl[t][v] = ok[t] ? X[((c0 + cl[v]) * Lc) * na + 3 * n + rr[t]] : 0.0f;
+        }
+    // w1 stays in this lane's registers; w2 goes to LDS for the first product
+    auto put_w = [&]() {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float a = 0.0f, b2 = 0.0f;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    a = a + f.a[b] * hs[t][v][b];
+                    b2 = b2 + f.b[b] * hs[t][v][b];
+                }
+                w1[t][v] = a;
+                w2buf[(4 * g + v) * PITCH + rr[t]] = ok[t] ? b2 : 0.0f;
+            }
+    };
+    // 16 chunks x NP padded states of `buf` times this wave's rows of A
+    auto product = [&](const float* buf, floatx4 (&acc)[2]) {
+        acc[0] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc[1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4* ab = reinterpret_cast<const floatx4*>(buf + l16 * PITCH + g * KS);
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+            const floatx4 a = ab[q];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bm[0][4 * q + i], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bm[1][4 * q + i], acc[1], 0, 0, 0);
+            }
+        }
+    };
+    put_w();
+    __syncthreads();
+    if (PASS2)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 4 * g + v;
+                const long e = (c0 + c) * Lc;
+                if (ok[t] && c < cReal && e < nReal) X[e * na + 3 * n + rr[t]] = hs[t][v][3];
+            }
+    const int steps = PASS2 ? Lc - 1 : Lc;
+    for (int j = 0; j < steps; ++j) {
+        float vn[2][4];
+        const bool more = j + 1 < steps || !PASS2;
+        const int jn = j + 1 < Lc ? j + 1 : Lc - 1;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                vn[t][v] = (ok[t] && more) ? X[((c0 + cl[v]) * Lc + jn) * na + 3 * n + rr[t]] : 0.0f;
+        floatx4 acc[2];
+        product(w2buf, acc);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                ubuf[(4 * g + v) * PITCH + rr[t]] = ok[t] ? w1[t][v] + acc[t][v] : 0.0f;
+        __syncthreads();
+        product(ubuf, acc);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 4 * g + v;
+                const float y = (acc[t][v] + vl[t][v]) - xc[t][v];
+                const float sum = xh[t][v] + y;
+                xc[t][v] = (sum - xh[t][v]) - y;
+                xh[t][v] = sum;
+                const float xv = sum - xc[t][v];
+#pragma unroll
+                for (int b = 0; b < 3; ++b) hs[t][v][b] = hs[t][v][b + 1];
+                hs[t][v][3] = xv;
+                if (PASS2 && ok[t]) {
+                    const long e = (c0 + c) * Lc + j + 1;
+                    if (c < cReal && e < nReal) X[e * na + 3 * n + rr[t]] = xv;
+                }
+                vl[t][v] = vn[t][v];
+            }
+        put_w();                          // every wave has read w2 this step
+        __syncthreads();
+    }
+    if (PASS2) return;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+        if (ok[t])
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 4 * g + v;
+                if (c < cReal)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        X[((c0 + c + 1) * Lc - 1) * na + b * n + rr[t]] = hs[t][v][b];
+            }
+}
+
+bool abm_chunks_factored_supported(int n) { return n >= 1 && n <= 256; }
 
 void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                          double h, const double* A64, const double* A264, float* X)
@@ -1854,14 +2018,32 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
             hipLaunchKernelGGL((k_abm_chunks_fac<NP, false>), grid, dim3(NP * 4), lds, s, n, lc,
                                nReal, A64, A264, f, X);
     };
+    auto launch_two = [&](auto np) {
+        constexpr int NP = decltype(np)::value;
+        const size_t lds = (size_t)2 * kChunkCols * (NP + 4) * sizeof(float);
+        if (pass2)
+            hipLaunchKernelGGL((k_abm_chunks_two<NP, true>), grid, dim3(NP * 2), lds, s, n, lc,
+                               nReal, A64, f, X);
+        else
+            hipLaunchKernelGGL((k_abm_chunks_two<NP, false>), grid, dim3(NP * 2), lds, s, n, lc,
+                               nReal, A64, f, X);
+    };
     if (n <= 16)
         launch(std::integral_constant<int, 16>());
     else if (n <= 32)
         launch(std::integral_constant<int, 32>());
     else if (n <= 64)
         launch(std::integral_constant<int, 64>());
-    else
+    else if (n <= 128)
         launch(std::integral_constant<int, 128>());
+    else if (n <= 160)
+        launch_two(std::integral_constant<int, 160>());
+    else if (n <= 192)
+        launch_two(std::integral_constant<int, 192>());
+    else if (n <= 224)
+        launch_two(std::integral_constant<int, 224>());
+    else
+        launch_two(std::integral_constant<int, 256>());
     check_launch("abm_chunks_factored");
 }
 
@@ -2049,7 +2231,9 @@ void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
 // the old X[R] (down; other tiles write X[R] rows meanwhile).  Tables: the
 // padded transposed FP32 table [nrp][nap] or the transposed FP64 power
 // [na][na].  Up: X[R] += P X[L]; down: X[L] = old X[R], X[R] = P old X[R] +
-// X[L].  Loads are clamped into range and issued eight k-steps ahead.
+// X[L]; with DELTA (tables P - I) the sum is the small part alone and the
+// epilogue adds it to X[R] + X[L] (up) or X[L] + old X[R] (down).  Loads are
+// clamped into range and issued eight k-steps ahead.
 template <typename TT>
 struct KsTile;
 template <>
@@ -2063,7 +2247,7 @@ struct KsTile<double> {
     using acc_t = doublex4;
 };
 
-template <typename TT, typename TX, bool DOWN>
+template <typename TT, typename TX, bool DOWN, bool DELTA>
 __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
                                                   const TT* __restrict__ PT,
                                                   const TX* __restrict__ S, TX* __restrict__ X)
@@ -2130,7 +2314,7 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
         const long col = (c0 + c) < np ? c0 + c : np - 1;
         const long R = (col + 1) * 2 * d - 1;
         xo[v] = X[(DOWN ? R - d : R) * na + icl];
-        xs[v] = DOWN ? S[col * na + icl] : (TX)0;
+        xs[v] = DOWN ? S[col * na + icl] : DELTA ? X[(R - d) * na + icl] : (TX)0;
     }
 #pragma unroll
     for (int v = 0; v < NACC; ++v) {
@@ -2141,10 +2325,10 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
         if (col >= np || i >= na) continue;
         const long R = (col + 1) * 2 * d - 1;
         if (!DOWN) {
-            X[R * na + i] = (TX)((TT)xo[v] + sum);
+            X[R * na + i] = DELTA ? (TX)((TT)xo[v] + ((TT)xs[v] + sum)) : (TX)((TT)xo[v] + sum);
         } else {
             X[(R - d) * na + i] = xs[v];
-            X[R * na + i] = (TX)(sum + (TT)xo[v]);
+            X[R * na + i] = DELTA ? (TX)((TT)xo[v] + ((TT)xs[v] + sum)) : (TX)(sum + (TT)xo[v]);
         }
     }
 }
@@ -2163,7 +2347,7 @@ __global__ void k_gather_rt(int na, long d, long np, const TX* __restrict__ X, T
 
 template <typename TT, typename TX>
 void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
-              TX* S)
+              TX* S, bool delta)
 {
     if (np <= 0) return;
     constexpr int T = KsTile<TT>::T;
@@ -2174,19 +2358,27 @@ void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const 
         if (!S) throw std::invalid_argument("level_ks: the down-sweep needs the copy buffer");
         hipLaunchKernelGGL((k_gather_rt<TX>), dim3(blocks_for(np * na, 256, 4096)), dim3(256), 0, s,
                            na, d, np, X, S);
-        hipLaunchKernelGGL((k_level_ks<TT, TX, true>), g, b, 0, s, na, ldp, d, np, PT, S, X);
+        if (delta)
+            hipLaunchKernelGGL((k_level_ks<TT, TX, true, true>), g, b, 0, s, na, ldp, d, np, PT, S,
+                               X);
+        else
+            hipLaunchKernelGGL((k_level_ks<TT, TX, true, false>), g, b, 0, s, na, ldp, d, np, PT,
+                               S, X);
+    } else if (delta) {
+        hipLaunchKernelGGL((k_level_ks<TT, TX, false, true>), g, b, 0, s, na, ldp, d, np, PT, S, X);
     } else {
-        hipLaunchKernelGGL((k_level_ks<TT, TX, false>), g, b, 0, s, na, ldp, d, np, PT, S, X);
+        hipLaunchKernelGGL((k_level_ks<TT, TX, false, false>), g, b, 0, s, na, ldp, d, np, PT, S,
+                           X);
     }
     check_launch("level_ks");
 }
 
 template void level_ks<float, float>(hipStream_t, bool, int, int, long, long, const float*, float*,
-                                     float*);
+                                     float*, bool);
 template void level_ks<double, float>(hipStream_t, bool, int, int, long, long, const double*,
-                                      float*, float*);
+                                      float*, float*, bool);
 template void level_ks<double, double>(hipStream_t, bool, int, int, long, long, const double*,
-                                       double*, double*);
+                                       double*, double*, bool);
 
 // ------------------------------------------------------- top of the tree
 // Levels [l0, L) of the up-sweep, the root and the same levels of the
@@ -2632,6 +2824,98 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
     }
 }
 
+// The same outputs for wide states: the states go through LDS in chunks of
+// kOutKc per element (64 x (kOutKc + 1) words per work-group, whatever n),
+// so many one-wave work-groups share a CU, and each lane keeps JG FP64 sums
+// (a group of outputs) over the chunks.  C^T and D^T rows are read with
+// scalar loads as above; the states are read coalesced (32 consecutive
+// states of two elements per load).
+constexpr int kOutKc = 32;
+
+template <typename TX, typename TU, int JG>
+__global__ __launch_bounds__(kOutLanes) void k_outputs_kc(
+    int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+    const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
+    const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
+{
+    __shared__ TX xs[kOutLanes * (kOutKc + 1)];
+    constexpr int XP = kOutKc + 1;
+    const int t = threadIdx.x, cl = t & (kOutKc - 1), r0 = t / kOutKc;
+    constexpr int RPL = kOutLanes / 2;    // staging loads per lane (two elements each)
+    static_assert(kOutLanes == 2 * kOutKc, "two elements per staging load");
+    const long stride = (long)gridDim.x * kOutLanes;
+    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += stride) {
+        const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
+        const long e = e0 + (t < cnt ? t : cnt - 1);
+        const TU* u = U + (e * sO + oO) * m;
+        for (int jb = 0; jb < pp; jb += JG) {
+            double v[JG];
+#pragma unroll
+            for (int jj = 0; jj < JG; ++jj) v[jj] = 0.0;
+            if (useC)
+                for (int kc = 0; kc < n; kc += kOutKc) {
+                    const int kn = (n - kc) < kOutKc ? n - kc : kOutKc;
+                    TX xv[RPL];
+#pragma unroll
+                    for (int q = 0; q < RPL; ++q) {
+                        const int r = r0 + 2 * q;
+                        const long er = e0 + (r < cnt ? r : cnt - 1);
+                        xv[q] = cl < kn ? X[er * na + (long)oX * n + kc + cl] : (TX)0;
+                    }
+                    __syncthreads();              // the previous chunk has been read
+#pragma unroll
+                    for (int q = 0; q < RPL; ++q) xs[(r0 + 2 * q) * XP + cl] = xv[q];
+                    __syncthreads();
+                    const TX* x = xs + t * XP;
+                    // RB rows of C^T per batch: at most 32 doubles in SGPRs
+                    constexpr int RB = JG > 16 ? 1 : 32 / JG;
+                    int c = 0;
+#pragma unroll 1
+                    for (; c + RB <= kn; c += RB) {
+                        double cr[RB][JG], xc[RB];
+#pragma unroll
+                        for (int q = 0; q < RB; ++q) {
+                            xc[q] = (double)x[c + q];
+#pragma unroll
+                            for (int jj = 0; jj < JG; ++jj)
+                                cr[q][jj] = CT[(long)(kc + c + q) * pp + jb + jj];
+                        }
+#pragma unroll
+                        for (int q = 0; q < RB; ++q)
+#pragma unroll
+                            for (int jj = 0; jj < JG; ++jj) v[jj] = fma(cr[q][jj], xc[q], v[jj]);
+                    }
+#pragma unroll 1
+                    for (; c < kn; ++c) {
+                        const double xc = (double)x[c];
+                        const double* cr = CT + (long)(kc + c) * pp + jb;
+#pragma unroll
+                        for (int jj = 0; jj < JG; ++jj) v[jj] = fma(cr[jj], xc, v[jj]);
+                    }
+                }
+            if (useD)
+                for (int c0 = 0; c0 < m; c0 += 8) {
+                    double uv[8];   // eight inputs in flight (indices past m repeat m - 1)
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) uv[cc] = (double)u[c0 + cc < m ? c0 + cc : m - 1];
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        if (c0 + cc >= m) break;
+                        const double* dr = DT + (long)(c0 + cc) * pp + jb;
+#pragma unroll
+                        for (int jj = 0; jj < JG; ++jj) v[jj] = fma(dr[jj], uv[cc], v[jj]);
+                    }
+                }
+            if (t < cnt) {
+                float* y = Y + e * p;
+#pragma unroll
+                for (int jj = 0; jj < JG; ++jj)
+                    if (jb + jj < p) y[jb + jj] = (float)v[jj];
+            }
+        }
+    }
+}
+
 __global__ void k_pad_cast(int rows, int cols, int pp, const float* __restrict__ in,
                            double* __restrict__ out)
 {
@@ -2659,13 +2943,31 @@ bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, b
     if (N <= 0 || p <= 0) return N <= 0;
     if (zC && zD) return false;   // the outputs are the states: k_outputs copies them
     if (pp % 8 != 0 || pp < p) throw std::invalid_argument("outputs_lanes: bad padding");
-    const size_t lds = (size_t)kOutLanes * (n + 1) * sizeof(TX);
-    if (lds > 65536) return false;
     // persistent work-groups (several blocks each)
     const long blocks = std::min<long>((N + kOutLanes - 1) / kOutLanes, 4096);
-    hipLaunchKernelGGL((k_outputs_lanes<TX, TU>), dim3((unsigned)std::max<long>(blocks, 1)),
-                       dim3(kOutLanes), lds, s, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U,
-                       sO, oO, Y);
+    const dim3 grid((unsigned)std::max<long>(blocks, 1));
+    if (!(sizeof(TX) == 4 && n <= kOutLanes)) {
+        // wide (or FP64) states: chunked through LDS, outputs in groups of JG
+        // that divide the padded count
+        auto go = [&](auto jg) {
+            constexpr int JG = decltype(jg)::value;
+            hipLaunchKernelGGL((k_outputs_kc<TX, TU, JG>), grid, dim3(kOutLanes), 0, s, n, m, p,
+                               pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
+        };
+        if (pp % 32 == 0)
+            go(std::integral_constant<int, 32>());
+        else if (pp % 24 == 0)
+            go(std::integral_constant<int, 24>());
+        else if (pp % 16 == 0)
+            go(std::integral_constant<int, 16>());
+        else
+            go(std::integral_constant<int, 8>());
+        check_launch("outputs_kc");
+        return true;
+    }
+    const size_t lds = (size_t)kOutLanes * (n + 1) * sizeof(TX);
+    hipLaunchKernelGGL((k_outputs_lanes<TX, TU>), grid, dim3(kOutLanes), lds, s, n, m, p, pp, na,
+                       oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
     check_launch("outputs_lanes");
     return true;
 }

@@ -112,9 +112,15 @@ bool ScanCore::level_is_f64(int l) const
     return l >= L_ - opt_.top64;
 }
 
+// Mixed precision, FP32 levels applied as X + (P - I) X: the lowest
+// opt_.deltaLevels, or all of them for wide states (n_a >= 768), where the
+// rounding of FP32 sums over n_a near-identity products would otherwise pass
+// the 1e-6 bar.
 bool ScanCore::level_is_delta(int l) const
 {
-    return opt_.precision == kMixed && !level_is_f64(l) && l < opt_.deltaLevels;
+    constexpr int kWideDeltaStates = 768;
+    return opt_.precision == kMixed && !level_is_f64(l) &&
+           (l < opt_.deltaLevels || na_ >= kWideDeltaStates);
 }
 
 // ------------------------------------------------------------------ step map
@@ -237,7 +243,7 @@ bool ScanCore::structured_level0(long nReal, long nPad) const
 }
 
 // The lowest levels as chunk recurrences (FP32 levels; RK4 n_a <= 256, ABM
-// n <= 64, or n <= 128 with an FP64 map): levels 0..lc-1 over chunks of 2^lc
+// n <= 64, or n <= 256 with an FP64 map): levels 0..lc-1 over chunks of 2^lc
 // elements (ABM: only the
 // companion map's dense block row per step).  lc is 7, or less so that a
 // launch has at least 1024 chunks (64 work-groups); at least one tree level
@@ -252,8 +258,10 @@ int ScanCore::chunk_log(long nReal, long nPad) const
     if (method_ == kABM ? !abmOk : !k::chunks_dense_supported(na_)) return 0;
     const int lc = std::min(kChunkLog, ilog2(nPad) - kMinChunksLog);
     if (lc < 2 || top_level(nReal, nPad) <= lc) return 0;
+    // (the chunk kernels read the FP64 map, so the FP32 tables' form, plain
+    // or delta, does not matter below lc)
     for (int l = 0; l < lc; ++l)
-        if (level_is_f64(l) || level_is_delta(l)) return 0;
+        if (level_is_f64(l)) return 0;
     return lc;
 }
 
@@ -354,7 +362,7 @@ void ScanCore::level(hipStream_t s, bool down, int l, long np)
 std::vector<int> ScanCore::level_candidates(int l, long np) const
 {
     std::vector<int> c;
-    const bool ks = np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l);
+    const bool ks = np <= k::kKsMaxPairs && na_ >= kKsMinStates;
     const bool f64 = xIsDouble() || level_is_f64(l);
     const bool mfma = !f64 && (level_kernel() & 1) == 0 && PTm_.get();
     if (ks && (f64 || mfma)) c.push_back(kPathKs);
@@ -371,7 +379,7 @@ void ScanCore::level_with(hipStream_t s, bool down, int l, long np, int path)
 {
     const size_t na2 = (size_t)na_ * na_;
     const long d = 1L << l;
-    const bool ksOk = np <= k::kKsMaxPairs && na_ >= kKsMinStates && !level_is_delta(l);
+    const bool ksOk = np <= k::kKsMaxPairs && na_ >= kKsMinStates;
     const bool mfmaOk = !xIsDouble() && !level_is_f64(l) && (level_kernel() & 1) == 0 && PTm_.get();
     int tile = 0;   // matrix-core tiles by the pair count
     if (path == kPathAuto && ksOk) path = kPathKs;
@@ -392,7 +400,7 @@ void ScanCore::level_with(hipStream_t s, bool down, int l, long np, int path)
         if (mfmaOk) {
             k::level_ks<float, float>(s, down, na_, nap_, d, np,
                                       PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(),
-                                      Sks32_.get());
+                                      Sks32_.get(), level_is_delta(l));
             return;
         }
     }

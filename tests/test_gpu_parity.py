@@ -300,6 +300,60 @@ def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
     assert err <= bound
 
 
+@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-5)])
+@pytest.mark.parametrize("solver,n,p", [(0, 200, 64), (1, 130, 40), (0, 90, 24), (1, 70, 17),
+                                        (0, 300, 5)])
+def test_wide_state_outputs_against_fp64(solver, n, p, mode, bound):
+    """Outputs of wide (n > 64) or FP64 states go through LDS in chunks of 32
+    states with the outputs in groups of 32, 24, 16 or 8 (whichever divides
+    the padded count): each group size, ragged chunks and ragged output
+    counts, against the FP64 oracle."""
+    s = synth.make_system(solver, n, 3, p, -6.0, -0.2, 1e-3, 1500, seed=900 + n)
+    U = synth.samples_f64(s)
+    api.set_precision(mode)
+    try:
+        tv, y, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h,
+                                            s.x0)
+    finally:
+        api.set_precision(api.PLAIN32)
+    assert e == 0, api.last_error_message()
+    assert y.shape == (1500, p)
+    y64 = orc.simulate_f64(solver, s.A.astype(np.float64), s.B.astype(np.float64),
+                           s.C.astype(np.float64), s.D.astype(np.float64),
+                           s.x0.astype(np.float64), U, s.steps, s.h)
+    err = orc.max_rel_err(y, y64)
+    print(f"mode {mode} solver {solver} n {n} p {p}: max rel err vs FP64 {err:.3e}")
+    assert err <= bound
+
+
+@pytest.mark.parametrize("n,steps", [(150, 5000), (200, 65541), (256, 3000)])
+def test_large_abm_chunk_recurrences(n, steps):
+    """ABM systems with 128 < n <= 256 in mixed precision run the lowest
+    levels as chunk recurrences whose step is two dependent products
+    (x3 + A (w1 + A w2) + v); padded widths 160, 224 and 256, chunk lengths
+    8, 128 and 4, against the FP64 oracle at the mixed-precision bar."""
+    s = synth.make_system(1, n, 4, 3, -6.0, -0.05, 1e-3, steps, seed=500 + n)
+    plan = api.Plan(1, n, 4, 3, 0, 0, steps, s.h, api.MIXED, True)
+    try:
+        lc = plan.chunk_log
+    finally:
+        plan.close()
+    assert lc == (7 if steps > 65536 else 2 if steps < 4096 else 3)
+    U = synth.samples_f64(s)
+    api.set_precision(api.MIXED)
+    try:
+        tv, y, e = api.simulate_ode_sampled(1, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h, s.x0)
+    finally:
+        api.set_precision(api.PLAIN32)
+    assert e == 0, api.last_error_message()
+    y64 = orc.simulate_f64(1, s.A.astype(np.float64), s.B.astype(np.float64),
+                           s.C.astype(np.float64), s.D.astype(np.float64),
+                           s.x0.astype(np.float64), U, s.steps, s.h)
+    err = orc.max_rel_err(y, y64)
+    print(f"n {n} N {steps} lc {lc}: max rel err vs FP64 {err:.3e}")
+    assert err <= 1e-6
+
+
 # ------------------------------------------- matrix-core vs vector kernels
 @pytest.mark.parametrize("solver,n", [(0, 20), (0, 48), (0, 100), (0, 256), (0, 300), (1, 17),
                                       (1, 40), (1, 64), (1, 130), (1, 256)])
