@@ -136,12 +136,98 @@ __global__ void k_gemm_f64_mfma(int r, int kk, int c, const double* __restrict__
 __global__ void k_gemm_f64_tiled(int r, int kk, int c, const double* __restrict__ X,
                                  const double* __restrict__ Y, double* __restrict__ Z);
 
+// FP64 GEMM for the squarings of wide power tables: 32 x 32 tiles of Z (so
+// n_a = 584 .. 1024 gives 361 .. 1024 work-groups), K split among KSPLIT
+// groups of four waves, each wave one 16 x 16 quadrant over every KSPLIT-th
+// k-step of a 32-deep slice staged in LDS; the groups' partial tiles meet in
+// LDS after the last slice and group 0 adds them in group order.  More waves
+// per SIMD than one-wave-per-tile kernels, whose dependent FP64 MFMAs leave the
+// pipe idle.  Loads are bounds-checked; the next slice is in flight while the
+// current one is multiplied.
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kGemmKsT = 32, kGemmKsKB = 32;
+
+template <int KSPLIT>
+__global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int c,
+                                                              const double* __restrict__ X,
+                                                              const double* __restrict__ Y,
+                                                              double* __restrict__ Z)
+{
+    constexpr int T = kGemmKsT, KB = kGemmKsKB, AP = KB + 1, BP = T + 1, NT = 256 * KSPLIT;
+    constexpr int PA = T * KB / NT, PB = KB * T / NT;
+    static_assert(PA >= 1 && PB >= 1 && 4 * KSPLIT <= KB, "tile, slice and split");
+    extern __shared__ double gsm[];
+    double* As = gsm;                      // [T][AP]
+    double* Bs = gsm + T * AP;             // [KB][BP]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = wave >> 2, wq = wave & 3;
+    const int wr = wq >> 1, wc = wq & 1, l16 = lane & 15, q4 = lane >> 4;
+    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
+    const int ar = tid / (KB / PA), ak = (tid % (KB / PA)) * PA;
+    const int bk = tid / (T / PB), bj = (tid % (T / PB)) * PB;
+    double ra[PA], rb[PB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < PA; ++q) {
+            const int gi = i0 + ar, gk = k0 + ak + q;
+            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int gk2 = k0 + bk, gj = j0 + bj + q;
+            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
+        }
+    };
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+    load(0);
+    for (int k0 = 0; k0 < kk; k0 += KB) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PA; ++q) As[ar * AP + ak + q] = ra[q];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) Bs[bk * BP + bj + q] = rb[q];
+        __syncthreads();
+        if (k0 + KB < kk) load(k0 + KB);
+#pragma unroll
+        for (int k4 = 4 * grp; k4 < KB; k4 += 4 * KSPLIT) {
+            const double a = As[(wr * 16 + l16) * AP + k4 + q4];
+            const double b = Bs[(k4 + q4) * BP + wc * 16 + l16];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+    }
+    __syncthreads();                      // the staging area holds the partial tiles now
+    double* part = gsm;                   // [KSPLIT - 1][4 waves][4 values][64 lanes]
+    if (grp > 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part[(((grp - 1) * 4 + wq) * 4 + v) * 64 + lane] = acc[v];
+    }
+    __syncthreads();
+    if (grp > 0) return;
+    // D[i][j] is in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double y = acc[v];
+#pragma unroll
+        for (int g = 1; g < KSPLIT; ++g) y += part[(((g - 1) * 4 + wq) * 4 + v) * 64 + lane];
+        const int i = i0 + wr * 16 + 4 * v + q4, j = j0 + wc * 16 + l16;
+        if (i < r && j < c) Z[(long)i * c + j] = y;
+    }
+}
+
 void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
               bool mfma)
 {
-    if (mfma && r >= 512 && c >= 512 && kk >= 64) {
-        dim3 grid((c + 63) / 64, (r + 63) / 64);
-        hipLaunchKernelGGL(k_gemm_f64_tiled, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
+    if (mfma && r >= 320 && c >= 320 && kk >= 64) {
+        const dim3 grid((c + kGemmKsT - 1) / kGemmKsT, (r + kGemmKsT - 1) / kGemmKsT);
+        const size_t stage = (size_t)(kGemmKsT * (kGemmKsKB + 1) + kGemmKsKB * (kGemmKsT + 1));
+        // four K groups while the tiles are few, two from 512 tiles on
+        if ((long)grid.x * grid.y < 512) {
+            const size_t lds = std::max<size_t>(stage, 3 * 4 * 4 * 64) * sizeof(double);
+            hipLaunchKernelGGL((k_gemm_f64_ks<4>), grid, dim3(1024), lds, s, r, kk, c, X, Y, Z);
+        } else {
+            const size_t lds = std::max<size_t>(stage, 1 * 4 * 4 * 64) * sizeof(double);
+            hipLaunchKernelGGL((k_gemm_f64_ks<2>), grid, dim3(512), lds, s, r, kk, c, X, Y, Z);
+        }
     } else if (mfma) {
         dim3 grid((c + 15) / 16, (r + 15) / 16);
         hipLaunchKernelGGL(k_gemm_f64_mfma, grid, dim3(256), 0, s, r, kk, c, X, Y, Z);
@@ -159,7 +245,6 @@ void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double
 // so a 256-deep product waits on one round of loads, not one per k-slice.  The
 // four partial tiles are summed through LDS.  Operands come from L2 (the
 // matrices are at most a few MiB).
-typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
                                                        const double* __restrict__ X,
