@@ -141,10 +141,13 @@ void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
 // FP64 MFMA when TT = double, with PT the transposed FP64 power, ldp = na).
 // The down-sweep first copies the pairs' X[R] into S (np * na values).
 constexpr long kKsMaxPairs = 4096;
-// delta: the table holds P - I (FP32 tables of mixed precision's delta levels)
+// delta: the table holds P - I (FP32 tables of mixed precision's delta levels).
+// Down-sweep: gathered = S already holds the pairs' X[R] (written by the level
+// above), else it is gathered first; Sn (not S), when given, receives the same
+// copy for the next lower level's npn pairs.
 template <typename TT, typename TX>
 void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
-              TX* S, bool delta = false);
+              TX* S, bool delta = false, bool gathered = false, TX* Sn = nullptr, long npn = 0);
 
 // Levels [l0, L) of both sweeps and the root in one work-group (levels >= l64
 // from the FP64 table, the others from the FP32 one).

@@ -133,8 +133,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int r, int kk, int c,
 
 __global__ void k_gemm_f64_mfma(int r, int kk, int c, const double* __restrict__ X,
                                 const double* __restrict__ Y, double* __restrict__ Z);
-__global__ void k_gemm_f64_tiled(int r, int kk, int c, const double* __restrict__ X,
-                                 const double* __restrict__ Y, double* __restrict__ Z);
 
 // FP64 GEMM for the squarings of wide power tables: 32 x 32 tiles of Z (so
 // n_a = 584 .. 1024 gives 361 .. 1024 work-groups), K split among KSPLIT
@@ -287,75 +285,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
             if (i < r && j < c) Z[(long)i * c + j] = sum;
         }
     }
-}
-
-// FP64 GEMM for large operands (the squarings of big power tables): a
-// work-group owns a 64 x 64 tile of Z, each wave a 32 x 32 quarter (2 x 2
-// MFMA tiles); 16-deep K-slices of X and Y pass through LDS, the next slice's
-// loads in flight in registers while the current one is multiplied.
-__global__ __launch_bounds__(256) void k_gemm_f64_tiled(int r, int kk, int c,
-                                                        const double* __restrict__ X,
-                                                        const double* __restrict__ Y,
-                                                        double* __restrict__ Z)
-{
-    constexpr int T = 64, KB = 16, AP = KB + 1, BP = T + 1;
-    __shared__ double As[T * AP];
-    __shared__ double Bs[KB * BP];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1, l16 = lane & 15, q4 = lane >> 4;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    // staging: A rows ar = tid / 4 (4 consecutive k each), B rows bk = tid / 16 (4 consecutive j)
-    const int ar = tid >> 2, ak = (tid & 3) * 4, bk = tid >> 4, bj = (tid & 15) * 4;
-    double ra[4], rb[4];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int gi = i0 + ar, gk = k0 + ak + q;
-            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
-            const int gk2 = k0 + bk, gj = j0 + bj + q;
-            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
-        }
-    };
-    doublex4 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = {0.0, 0.0, 0.0, 0.0};
-    load(0);
-    for (int k0 = 0; k0 < kk; k0 += KB) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            As[ar * AP + ak + q] = ra[q];
-            Bs[bk * BP + bj + q] = rb[q];
-        }
-        __syncthreads();
-        if (k0 + KB < kk) load(k0 + KB);
-#pragma unroll
-        for (int k4 = 0; k4 < KB; k4 += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                a[t] = As[(wr * 32 + t * 16 + l16) * AP + k4 + q4];
-                b[t] = Bs[(k4 + q4) * BP + wc * 32 + t * 16 + l16];
-            }
-#pragma unroll
-            for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-                for (int tj = 0; tj < 2; ++tj)
-                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
-        }
-    }
-    // D[i][j] is in lane j + 16 (i % 4), register i / 4
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + wr * 32 + ti * 16 + 4 * v + q4, j = j0 + wc * 32 + tj * 16 + l16;
-                if (i < r && j < c) Z[(long)i * c + j] = acc[ti][tj][v];
-            }
 }
 
 __global__ void k_cast_f32_f64(long count, const float* __restrict__ in, double* __restrict__ out)
@@ -2335,7 +2264,8 @@ struct KsTile<double> {
 template <typename TT, typename TX, bool DOWN, bool DELTA>
 __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
                                                   const TT* __restrict__ PT,
-                                                  const TX* __restrict__ S, TX* __restrict__ X)
+                                                  const TX* __restrict__ S, TX* __restrict__ X,
+                                                  TX* __restrict__ Sn, long npn)
 {
     using K = KsTile<TT>;
     constexpr int T = K::T, KS = K::KS, NACC = K::NACC, U = 8;
@@ -2412,8 +2342,15 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
         if (!DOWN) {
             X[R * na + i] = DELTA ? (TX)((TT)xo[v] + ((TT)xs[v] + sum)) : (TX)((TT)xo[v] + sum);
         } else {
+            const TX y = DELTA ? (TX)((TT)xo[v] + ((TT)xs[v] + sum)) : (TX)(sum + (TT)xo[v]);
             X[(R - d) * na + i] = xs[v];
-            X[R * na + i] = DELTA ? (TX)((TT)xo[v] + ((TT)xs[v] + sum)) : (TX)(sum + (TT)xo[v]);
+            X[R * na + i] = y;
+            // the next (lower) level's copy of its pairs' X[R]: its pairs 2 col
+            // and 2 col + 1 end at this pair's L and R
+            if (Sn) {
+                if (2 * col < npn) Sn[(2 * col) * na + i] = xs[v];
+                if (2 * col + 1 < npn) Sn[(2 * col + 1) * na + i] = y;
+            }
         }
     }
 }
@@ -2432,7 +2369,7 @@ __global__ void k_gather_rt(int na, long d, long np, const TX* __restrict__ X, T
 
 template <typename TT, typename TX>
 void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const TT* PT, TX* X,
-              TX* S, bool delta)
+              TX* S, bool delta, bool gathered, TX* Sn, long npn)
 {
     if (np <= 0) return;
     constexpr int T = KsTile<TT>::T;
@@ -2441,29 +2378,33 @@ void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const 
     dim3 g((unsigned)((np + T - 1) / T), (unsigned)((na + T - 1) / T)), b(256);
     if (down) {
         if (!S) throw std::invalid_argument("level_ks: the down-sweep needs the copy buffer");
-        hipLaunchKernelGGL((k_gather_rt<TX>), dim3(blocks_for(np * na, 256, 4096)), dim3(256), 0, s,
-                           na, d, np, X, S);
+        if (Sn && (Sn == S || npn < 0 || npn > 2 * np))
+            throw std::invalid_argument("level_ks: bad next-level copy");
+        if (!gathered)
+            hipLaunchKernelGGL((k_gather_rt<TX>), dim3(blocks_for(np * na, 256, 4096)), dim3(256),
+                               0, s, na, d, np, X, S);
         if (delta)
             hipLaunchKernelGGL((k_level_ks<TT, TX, true, true>), g, b, 0, s, na, ldp, d, np, PT, S,
-                               X);
+                               X, Sn, npn);
         else
             hipLaunchKernelGGL((k_level_ks<TT, TX, true, false>), g, b, 0, s, na, ldp, d, np, PT,
-                               S, X);
+                               S, X, Sn, npn);
     } else if (delta) {
-        hipLaunchKernelGGL((k_level_ks<TT, TX, false, true>), g, b, 0, s, na, ldp, d, np, PT, S, X);
+        hipLaunchKernelGGL((k_level_ks<TT, TX, false, true>), g, b, 0, s, na, ldp, d, np, PT, S, X,
+                           nullptr, 0L);
     } else {
         hipLaunchKernelGGL((k_level_ks<TT, TX, false, false>), g, b, 0, s, na, ldp, d, np, PT, S,
-                           X);
+                           X, nullptr, 0L);
     }
     check_launch("level_ks");
 }
 
 template void level_ks<float, float>(hipStream_t, bool, int, int, long, long, const float*, float*,
-                                     float*, bool);
+                                     float*, bool, bool, float*, long);
 template void level_ks<double, float>(hipStream_t, bool, int, int, long, long, const double*,
-                                      float*, float*, bool);
+                                      float*, float*, bool, bool, float*, long);
 template void level_ks<double, double>(hipStream_t, bool, int, int, long, long, const double*,
-                                       double*, double*, bool);
+                                       double*, double*, bool, bool, double*, long);
 
 // ------------------------------------------------------- top of the tree
 // Levels [l0, L) of the up-sweep, the root and the same levels of the

@@ -58,11 +58,11 @@ ScanCore::ScanCore(int method, int n, int m, int p, bool zC, bool zD, long nPadM
         X32_.alloc((size_t)nPadMax_ * na_, arena);
     // down-sweep copy of X[R] for the K-split levels with few pairs
     if (na_ >= kKsMinStates) {
-        const size_t ks = (size_t)std::min<long>(k::kKsMaxPairs, nPadMax_ / 2 > 0 ? nPadMax_ / 2 : 1) * na_;
+        ksHalf_ = (size_t)std::min<long>(k::kKsMaxPairs, nPadMax_ / 2 > 0 ? nPadMax_ / 2 : 1) * na_;
         if (xIsDouble())
-            Sks64_.alloc(ks, arena);
+            Sks64_.alloc(2 * ksHalf_, arena);
         else
-            Sks32_.alloc(ks, arena);
+            Sks32_.alloc(2 * ksHalf_, arena);
     }
     seed_.alloc(na_, arena);
     tot_.alloc(na_, arena);
@@ -89,7 +89,7 @@ size_t ScanCore::estimate_footprint(int method, int n, int m, long nPadMax, Scan
     }
     b += (opt.precision == kFp64 ? 8 : 4) * (size_t)nP * na;
     if ((int)na >= kKsMinStates)
-        b += (opt.precision == kFp64 ? 8 : 4) *
+        b += 2 * (opt.precision == kFp64 ? 8 : 4) *
              (size_t)std::min<long>(k::kKsMaxPairs, nP / 2 > 0 ? nP / 2 : 1) * na;
     b += 24 * na;
     return b + 20 * Arena::kAlign;   // block rounding when carved from an Arena
@@ -387,20 +387,29 @@ void ScanCore::level_with(hipStream_t s, bool down, int l, long np, int path)
     if (path == kPathWide) tile = 2;
     // few pairs: small K-split tiles on the matrix cores, spread over many CUs
     if (path == kPathKs && ksOk) {
+        // down: this level's copy of X[R] in one half (gathered first unless
+        // the level above wrote it), the next level's, when it is K-split too,
+        // into the other
+        const size_t in = down ? (size_t)ksc_.half * ksHalf_ : 0;
+        const size_t out = (size_t)(1 - ksc_.half) * ksHalf_;
+        const bool gathered = down && ksc_.gathered, chain = down && ksc_.out;
         if (xIsDouble()) {
             k::level_ks<double, double>(s, down, na_, na_, d, np, PT64_.get() + l * na2,
-                                        X64_.get(), Sks64_.get());
+                                        X64_.get(), Sks64_.get() + in, false, gathered,
+                                        chain ? Sks64_.get() + out : nullptr, ksc_.npOut);
             return;
         }
         if (level_is_f64(l)) {
             k::level_ks<double, float>(s, down, na_, na_, d, np, PT64_.get() + l * na2,
-                                       X32_.get(), Sks32_.get());
+                                       X32_.get(), Sks32_.get() + in, false, gathered,
+                                       chain ? Sks32_.get() + out : nullptr, ksc_.npOut);
             return;
         }
         if (mfmaOk) {
             k::level_ks<float, float>(s, down, na_, nap_, d, np,
                                       PTm_.get() + (size_t)l * nap_ * nrp_, X32_.get(),
-                                      Sks32_.get(), level_is_delta(l));
+                                      Sks32_.get() + in, level_is_delta(l), gathered,
+                                      chain ? Sks32_.get() + out : nullptr, ksc_.npOut);
             return;
         }
     }
@@ -515,6 +524,7 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
     const bool s1 = s0 && structured_level1(nReal, nPad);
     const int lc = chunk_log(nReal, nPad);
+    const int lo = lc > 0 ? lc : s1 ? 2 : s0 ? 1 : 0;   // the lowest tree level
     hipEvent_t e0 = nullptr, e1 = nullptr;
     PARODE_HIP(hipEventCreate(&e0));
     PARODE_HIP(hipEventCreate(&e1));
@@ -530,6 +540,17 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
                 float bestMs = 0.0f;
                 for (int c : cand) {
                     constexpr int kReps = 3;
+                    // a down-sweep K-split level is timed as inside a chain of
+                    // them: its copy of X[R] written by the level above (when
+                    // there is one), the next level's copy written by it
+                    // (when there is one); the copies' contents do not matter here
+                    ksc_ = KsChain{};
+                    if (down && c == kPathKs) {
+                        const long npNext = l - 1 >= lo ? pairs_at(nReal, l - 1) : 0;
+                        ksc_.gathered = l + 1 < lt;
+                        ksc_.out = npNext > 0 && npNext <= k::kKsMaxPairs;
+                        ksc_.npOut = ksc_.out ? npNext : 0;
+                    }
                     // a long fill first, so that every timed launch is queued
                     // before the GPU reaches it: the events then time the GPU
                     // alone, as in a solve, not the host's launch rate
@@ -538,6 +559,7 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
                     PARODE_HIP(hipEventRecord(e0, s));
                     for (int r = 0; r < kReps; ++r) level_with(s, down != 0, l, np, c);
                     PARODE_HIP(hipEventRecord(e1, s));
+                    ksc_ = KsChain{};
                     PARODE_HIP(hipEventSynchronize(e1));
                     float ms = 0.0f;
                     PARODE_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -552,6 +574,7 @@ int ScanCore::autotune(hipStream_t s, long nReal, long nPad)
             }
         }
     } catch (...) {
+        ksc_ = KsChain{};
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         throw;
@@ -580,17 +603,48 @@ void ScanCore::sweep_top(hipStream_t s, long nReal, long nPad, const double* dSe
                       dTot);
 }
 
+// Whether level_with(path) runs level l on the K-split kernel.
+bool ScanCore::ks_taken(int l, long np, int path) const
+{
+    if (np > k::kKsMaxPairs || na_ < kKsMinStates) return false;
+    if (path != kPathAuto && path != kPathKs) return false;
+    const bool mfmaOk = !xIsDouble() && !level_is_f64(l) && (level_kernel() & 1) == 0 && PTm_.get();
+    return xIsDouble() || level_is_f64(l) || mfmaOk;
+}
+
+void ScanCore::down_levels(hipStream_t s, int hi, int lo, long nReal)
+{
+    bool haveCopy = false;
+    int half = 0;
+    for (int l = hi; l >= lo; --l) {
+        const long np = pairs_at(nReal, l);
+        const int path = scheduled_path(true, l, np);
+        const bool ks = ks_taken(l, np, path);
+        bool next = false;
+        long npNext = 0;
+        if (ks && l - 1 >= lo) {
+            npNext = pairs_at(nReal, l - 1);
+            next = ks_taken(l - 1, npNext, scheduled_path(true, l - 1, npNext));
+        }
+        ksc_ = KsChain{ks && haveCopy, half, next, next ? npNext : 0};
+        level_with(s, true, l, np, path);
+        haveCopy = next;
+        if (next) half ^= 1;
+    }
+    ksc_ = KsChain{};
+}
+
 void ScanCore::sweep_down(hipStream_t s, long nReal, long nPad)
 {
     const int lt = top_level(nReal, nPad);
     if (const int lc = chunk_log(nReal, nPad)) {   // levels below lc: chunk recurrences
-        for (int l = lt - 1; l >= lc; --l) level(s, true, l, pairs_at(nReal, l));
+        down_levels(s, lt - 1, lc, nReal);
         chunks(s, true, nReal, nPad);
         return;
     }
     const bool s0 = lt > 0 && structured_level0(nReal, nPad);
     const bool s1 = s0 && structured_level1(nReal, nPad);
-    for (int l = lt - 1; l >= (s1 ? 2 : s0 ? 1 : 0); --l) level(s, true, l, pairs_at(nReal, l));
+    down_levels(s, lt - 1, s1 ? 2 : s0 ? 1 : 0, nReal);
     if (s1)
         k::level_mfma(s, true, false, na_, 2, pairs_at(nReal, 1), PTm_.get() + (size_t)nap_ * nrp_,
                       X32_.get(), abm_rows(n_, 1, true, false), Sg_.get());

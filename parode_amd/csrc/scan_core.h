@@ -125,6 +125,10 @@ private:
     bool level_is_delta(int l) const;
     void level(hipStream_t s, bool down, int l, long np);
     void level_with(hipStream_t s, bool down, int l, long np, int path);
+    // down-sweep levels hi .. lo (descending); consecutive K-split levels pass
+    // the copy of their pairs' X[R] down without a gather launch
+    void down_levels(hipStream_t s, int hi, int lo, long nReal);
+    bool ks_taken(int l, long np, int path) const;
     // the chunk recurrences of the lowest levels (pass 1 or 2)
     void chunks(hipStream_t s, bool pass2, long nReal, long nPad);
     std::vector<int> level_candidates(int l, long np) const;
@@ -153,8 +157,15 @@ private:
     DevArray<float> PT32_;
     DevArray<float> PTm_;            // zero-padded FP32 tables of the MFMA levels
     DevArray<float> Sg_;             // X[R] copy for row-split down-sweep levels (na > 256)
-    DevArray<float> Sks32_;          // X[R] copy for K-split down-sweep levels (few pairs)
-    DevArray<double> Sks64_;
+    DevArray<float> Sks32_;          // X[R] copies for K-split down-sweep levels (few
+    DevArray<double> Sks64_;         // pairs): two halves of ksHalf_ values, ping-pong
+    size_t ksHalf_ = 0;
+    struct KsChain {                 // the down-sweep level being launched
+        bool gathered = false;       // its half already holds the copy
+        int half = 0;
+        bool out = false;            // it writes the next level's copy (other half)
+        long npOut = 0;
+    } ksc_;
     int nap_ = 0, nrp_ = 0;
     // trajectory
     DevArray<float> X32_;
