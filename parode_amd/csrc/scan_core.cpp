@@ -662,20 +662,33 @@ void ScanCore::carry(hipStream_t s, long nPad, const double* dSeed, const double
 }
 
 // -------------------------------------------------------------------- outputs
-void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const void* dU,
-                       bool uDouble, long sO, long oO, long N, float* dY)
+// C^T and D^T (and their FP64 copies padded to pp columns) for the output
+// kernels: they depend only on C and D, so a solve makes them on its side
+// stream, off the path to the outputs.
+void ScanCore::prepare_outputs(hipStream_t s, const float* dC, const float* dD, bool uDouble)
 {
-    const int oX = method_ == kABM ? 3 : 0;
     const bool zC = zC_ || !dC, zD = zD_ || !dD;
     if (!zC) k::transpose<float>(s, p_, n_, dC, CT_.get());
     if (!zD && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
+    if (xIsDouble() || uDouble) {
+        const int pp = pad8(p_);
+        if (!zC) k::pad_cast(s, n_, p_, pp, CT_.get(), CT64_.get());
+        if (!zD && m_ > 0) k::pad_cast(s, m_, p_, pp, DT_.get(), DT64_.get());
+    }
+}
+
+// prepared: prepare_outputs' tables were made earlier on this stream or on one
+// joined to it; else they are made here first.
+void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const void* dU,
+                       bool uDouble, long sO, long oO, long N, float* dY, bool prepared)
+{
+    const int oX = method_ == kABM ? 3 : 0;
+    if (!prepared) prepare_outputs(s, dC, dD, uDouble);
     dC = CT_.get();
     dD = DT_.get();
     // FP64 outputs: one thread per element where the shape fits
     if (xIsDouble() || uDouble) {
         const int pp = pad8(p_);
-        if (!zC) k::pad_cast(s, n_, p_, pp, CT_.get(), CT64_.get());
-        if (!zD && m_ > 0) k::pad_cast(s, m_, p_, pp, DT_.get(), DT64_.get());
         bool done;
         if (xIsDouble())
             done = uDouble ? k::outputs_lanes<double, double>(s, n_, m_, p_, pp, na_, oX, zC_, zD_,
@@ -709,12 +722,14 @@ void ScanCore::outputs(hipStream_t s, const float* dC, const float* dD, const vo
 }
 
 void ScanCore::startup_outputs(hipStream_t s, const float* dC, const float* dD, const double* dX0,
-                               const void* dU, bool uDouble, float* dY)
+                               const void* dU, bool uDouble, float* dY, bool prepared)
 {
     // x_0, x_1, x_2 are the first three n-blocks of X0; their inputs are
     // rows 0, 2, 4 of the h/2-grid start-up samples (AdamsMoulton.cpp:106-116)
-    if (!zC_) k::transpose<float>(s, p_, n_, dC, CT_.get());
-    if (!zD_ && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
+    if (!prepared) {
+        if (!zC_) k::transpose<float>(s, p_, n_, dC, CT_.get());
+        if (!zD_ && m_ > 0) k::transpose<float>(s, p_, m_, dD, DT_.get());
+    }
     if (uDouble)
         k::outputs<double, double>(s, n_, m_, p_, n_, 0, zC_, zD_, 3, CT_.get(), DT_.get(), dX0,
                                    (const double*)dU, 2, 0, dY);
@@ -752,12 +767,13 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
     build_map(s, dA, dB, h, side);
     hipStream_t ps = side ? side->stream : s;
     build_powers(ps);
+    prepare_outputs(ps, dC, dD, uDouble);
     float* yScan = dY;
     if (method_ == kRK4) {
         k::to_f64<float>(ps, n_, dx0, seed());
     } else {
         abm_startup(ps, dA, dB, dx0, dU, uDouble, h, seed());
-        startup_outputs(ps, dC, dD, seed(), dU, uDouble, dY);
+        startup_outputs(ps, dC, dD, seed(), dU, uDouble, dY, true);
         yScan = dY + 3L * nOut();
     }
     if (side) PARODE_HIP(hipEventRecord(side->join, ps));
@@ -773,7 +789,7 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
     tick(4);
     sweep_down(s, nScan, nPad);
     tick(5);
-    outputs(s, dC, dD, dU, uDouble, 2, method_ == kRK4 ? 0 : 6, nScan, yScan);
+    outputs(s, dC, dD, dU, uDouble, 2, method_ == kRK4 ? 0 : 6, nScan, yScan, true);
     tick(6);
 }
 

@@ -75,11 +75,14 @@ public:
     void sweep_down(hipStream_t s, long nReal, long nPad);
     // seedOut = M^nPad * seed + tot (the accumulator carried to the next batch)
     void carry(hipStream_t s, long nPad, const double* dSeed, const double* dTot, double* dOut);
+    // C^T, D^T and their padded FP64 copies for the output kernels
+    void prepare_outputs(hipStream_t s, const float* dC, const float* dD, bool uDouble);
+    // prepared: prepare_outputs ran before on this stream (or one joined to it)
     void outputs(hipStream_t s, const float* dC, const float* dD, const void* dU, bool uDouble,
-                 long sO, long oO, long N, float* dY);
+                 long sO, long oO, long N, float* dY, bool prepared = false);
     // outputs of the ABM start-up states x_0..x_2 (dX0 holds x_0..x_3)
     void startup_outputs(hipStream_t s, const float* dC, const float* dD, const double* dX0,
-                         const void* dU, bool uDouble, float* dY);
+                         const void* dU, bool uDouble, float* dY, bool prepared = false);
     void abm_startup(hipStream_t s, const float* dA, const float* dB, const float* dx0,
                      const void* dUh, bool uDouble, double h, double* dX0);
 
