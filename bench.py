@@ -245,10 +245,12 @@ def batch_legs(args, torch, api, synth):
     return out, systems, ys, y_plain
 
 
-def batch_accuracy(systems, ys, y_plain, count):
+def batch_accuracy(systems, ys, y_plain, count, wide):
     """Max relative error vs the FP64 oracle over a fixed sample of the served
-    results (every system when count >= len(systems)), and the plain results
-    of the first systems vs their served ones."""
+    results: the `count` cheapest systems to simulate (every system when count
+    >= len(systems)) and, apart, the `wide` cheapest with n_a >= 768 (where the
+    delta levels and, for ABM, the two-product chunk step run); and the plain
+    results of the first systems vs their served ones."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     from parode_amd import synth
@@ -261,8 +263,17 @@ def batch_accuracy(systems, ys, y_plain, count):
         y64 = orc.simulate_f64(s.method, f(s.A), f(s.B), f(s.C), f(s.D), f(s.x0),
                                synth.samples_f64(s), s.steps, s.h)
         worst = max(worst, orc.max_rel_err(ys[i], y64))
+    wide_idx = [i for i in idx if systems[i].n_scan >= 768][:max(0, wide)]
+    worst_wide = 0.0
+    for i in wide_idx:
+        s = systems[i]
+        y64 = orc.simulate_f64(s.method, f(s.A), f(s.B), f(s.C), f(s.D), f(s.x0),
+                               synth.samples_f64(s), s.steps, s.h)
+        worst_wide = max(worst_wide, orc.max_rel_err(ys[i], y64))
     plain_vs_served = max(orc.max_rel_err(yp, f(ys[i])) for i, yp in enumerate(y_plain))
     return {"served_max_rel_err_vs_fp64": worst, "checked_systems": len(pick),
+            "served_max_rel_err_vs_fp64_wide": worst_wide if wide_idx else None,
+            "checked_wide_systems": [[systems[i].n, systems[i].steps] for i in wide_idx],
             "plain_vs_served_max_rel": plain_vs_served}
 
 
@@ -285,6 +296,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--stream-seconds", type=float, default=10.0)
     ap.add_argument("--check-systems", type=int, default=48)
+    ap.add_argument("--check-wide", type=int, default=8,
+                    help="served systems with n_a >= 768 also checked against the FP64 oracle")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device to device (counter calibration)")
     args = ap.parse_args()
@@ -404,7 +417,7 @@ def main():
     if args.batch > 0:
         bl, systems, ys, y_plain = batch_legs(args, torch, api, synth)
         if not args.no_check:
-            bl.update(batch_accuracy(systems, ys, y_plain, args.check_systems))
+            bl.update(batch_accuracy(systems, ys, y_plain, args.check_systems, args.check_wide))
         line["batch"] = bl
     if not args.no_cpu or not args.no_check:
         cb, acc = cpu_baseline_leg(s, U64, y, full_reference=not args.no_check)
