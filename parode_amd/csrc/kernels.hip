@@ -792,7 +792,7 @@ constexpr int kLeafTiles = 4;
 constexpr int kLeafMaxQ = 5, kLeafMaxG = 2;
 
 template <typename T, typename TU, bool FAC>
-__global__ __launch_bounds__(256) void k_leaves_mfma(int n, int m, int kq, int na, int off,
+__global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, int na, int off,
                                                      long nReal, long nPad, int groups,
                                                      int pitch, unsigned magic, int kg,
                                                      LeafComb G, bool writeZeros, int zlo,
