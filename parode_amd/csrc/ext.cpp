@@ -40,6 +40,7 @@ struct Plan {
             if (e) (void)hipEventDestroy(e);
         if (side.fork) (void)hipEventDestroy(side.fork);
         if (side.join) (void)hipEventDestroy(side.join);
+        if (side.late) (void)hipEventDestroy(side.late);
         if (side.stream) (void)hipStreamDestroy(side.stream);
     }
     const parode::ScanCore::SideStream* side_stream()
@@ -48,6 +49,7 @@ struct Plan {
             PARODE_HIP(hipStreamCreate(&side.stream));
             PARODE_HIP(hipEventCreateWithFlags(&side.fork, hipEventDisableTiming));
             PARODE_HIP(hipEventCreateWithFlags(&side.join, hipEventDisableTiming));
+            PARODE_HIP(hipEventCreateWithFlags(&side.late, hipEventDisableTiming));
         }
         return &side;
     }

@@ -765,8 +765,21 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
     // the power table and the ABM start-up (and for an FP64 ABM map, the
     // map's last launches): on the side stream when given
     build_map(s, dA, dB, h, side);
+    tick(1);
+    // the leaves read only what the map's launches on this stream made: they
+    // are queued before the side stream's many launches, so this stream does
+    // not wait for the host to queue those
+    if (method_ == kRK4)
+        leaves(s, dU, uDouble, 2, 0, 1, nScan, nPad);
+    else
+        leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
+    tick(2);
     hipStream_t ps = side ? side->stream : s;
     build_powers(ps);
+    // the tree levels wait for the tables only; the root for the seed, the
+    // outputs for their tables
+    const bool late = side && side->late;
+    if (late) PARODE_HIP(hipEventRecord(side->join, ps));
     prepare_outputs(ps, dC, dD, uDouble);
     float* yScan = dY;
     if (method_ == kRK4) {
@@ -776,15 +789,10 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
         startup_outputs(ps, dC, dD, seed(), dU, uDouble, dY, true);
         yScan = dY + 3L * nOut();
     }
-    if (side) PARODE_HIP(hipEventRecord(side->join, ps));
-    tick(1);
-    if (method_ == kRK4)
-        leaves(s, dU, uDouble, 2, 0, 1, nScan, nPad);
-    else
-        leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
-    tick(2);
+    if (side) PARODE_HIP(hipEventRecord(late ? side->late : side->join, ps));
     sweep_up(s, nScan, nPad, side ? side->join : nullptr);
     tick(3);
+    if (late) PARODE_HIP(hipStreamWaitEvent(s, side->late, 0));
     sweep_top(s, nScan, nPad, seed(), tot());
     tick(4);
     sweep_down(s, nScan, nPad);

@@ -55,9 +55,12 @@ public:
 
     // With side, the launches that the leaves and the chunk recurrences do not
     // need are forked onto side->stream (fork event recorded on s).
+    // join: the level tables are ready (waited before the first tree level);
+    // late (optional): the start-up seed and the output tables are ready
+    // (waited before the root), else join covers them too
     struct SideStream {
         hipStream_t stream = nullptr;
-        hipEvent_t fork = nullptr, join = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr, late = nullptr;
     };
     void build_map(hipStream_t s, const float* dA, const float* dB, double h,
                    const SideStream* side = nullptr);
