@@ -2248,27 +2248,38 @@ void abm_chunks(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
 // X[L]; with DELTA (tables P - I) the sum is the small part alone and the
 // epilogue adds it to X[R] + X[L] (up) or X[L] + old X[R] (down).  Loads are
 // clamped into range and issued eight k-steps ahead.
-template <typename TT>
+// SMALL (FP32 only): 16 x 16 tiles on v_mfma_f32_16x16x4f32, four times the
+// work-groups of the 32 x 32 tiles, for levels whose 32 x 32 grid would leave
+// most CUs idle.
+template <typename TT, bool SMALL>
 struct KsTile;
 template <>
-struct KsTile<float> {
+struct KsTile<float, false> {
     static constexpr int T = 32, KS = 2, NACC = 16;
     using acc_t = floatx16;
 };
 template <>
-struct KsTile<double> {
+struct KsTile<float, true> {
+    static constexpr int T = 16, KS = 4, NACC = 4;
+    using acc_t = floatx4;
+};
+template <>
+struct KsTile<double, false> {
     static constexpr int T = 16, KS = 4, NACC = 4;
     using acc_t = doublex4;
 };
 
-template <typename TT, typename TX, bool DOWN, bool DELTA>
+template <typename TT, typename TX, bool DOWN, bool DELTA, bool SMALL>
 __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
                                                   const TT* __restrict__ PT,
                                                   const TX* __restrict__ S, TX* __restrict__ X,
                                                   TX* __restrict__ Sn, long npn)
 {
-    using K = KsTile<TT>;
+    using K = KsTile<TT, SMALL>;
     constexpr int T = K::T, KS = K::KS, NACC = K::NACC, U = 8;
+    // fragment and result layouts: 32 x 32 x 2 FP32 (T = 32), or the 16 x 16 x 4
+    // shapes (T = 16), whose A and B layouts agree and whose D layouts differ
+    constexpr bool F32BIG = sizeof(TT) == 4 && !SMALL;
     __shared__ TT part[3][NACC][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long c0 = (long)blockIdx.x * T;
@@ -2306,8 +2317,10 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if constexpr (sizeof(TT) == 4)
+            if constexpr (F32BIG)
                 acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+            else if constexpr (sizeof(TT) == 4)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
             else
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
@@ -2318,14 +2331,15 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
     }
     __syncthreads();
     if (wave != 0) return;
-    // D: FP32 (c = 8(v/4) + 4(l/32) + v%4, i = l%32); FP64 (c = 4v + l/16, i = l%16).
+    // D: FP32 32x32 (c = 8(v/4) + 4(l/32) + v%4, i = l%32); FP32 16x16
+    // (c = 4(l/16) + v, i = l%16); FP64 (c = 4v + l/16, i = l%16).
     // All loads first (clamped addresses), then the guarded stores.
     const int i = i0 + lc, icl = i < na ? i : na - 1;
     TX xo[NACC], xs[NACC];
 #pragma unroll
     for (int v = 0; v < NACC; ++v) {
-        const int c = (sizeof(TT) == 4) ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
-                                        : 4 * v + (lane >> 4);
+        const int c = F32BIG ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
+                             : sizeof(TT) == 4 ? 4 * (lane >> 4) + v : 4 * v + (lane >> 4);
         const long col = (c0 + c) < np ? c0 + c : np - 1;
         const long R = (col + 1) * 2 * d - 1;
         xo[v] = X[(DOWN ? R - d : R) * na + icl];
@@ -2334,8 +2348,8 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
 #pragma unroll
     for (int v = 0; v < NACC; ++v) {
         const TT sum = acc[v] + part[0][v][lane] + part[1][v][lane] + part[2][v][lane];
-        const int c = (sizeof(TT) == 4) ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
-                                        : 4 * v + (lane >> 4);
+        const int c = F32BIG ? 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)
+                             : sizeof(TT) == 4 ? 4 * (lane >> 4) + v : 4 * v + (lane >> 4);
         const long col = c0 + c;
         if (col >= np || i >= na) continue;
         const long R = (col + 1) * 2 * d - 1;
@@ -2372,9 +2386,11 @@ void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const 
               TX* S, bool delta, bool gathered, TX* Sn, long npn)
 {
     if (np <= 0) return;
-    constexpr int T = KsTile<TT>::T;
     if (sizeof(TT) == 4 && ldp % 32 != 0)
         throw std::invalid_argument("level_ks: FP32 table must be the padded MFMA table");
+    // FP32: 16 x 16 tiles while 32 x 32 ones would make fewer than 1024 work-groups
+    const bool small = sizeof(TT) == 4 && ((np + 31) / 32) * ((na + 31) / 32) < 1024;
+    const int T = sizeof(TT) == 4 && !small ? 32 : 16;
     dim3 g((unsigned)((np + T - 1) / T), (unsigned)((na + T - 1) / T)), b(256);
     if (down) {
         if (!S) throw std::invalid_argument("level_ks: the down-sweep needs the copy buffer");
@@ -2383,18 +2399,25 @@ void level_ks(hipStream_t s, bool down, int na, int ldp, long d, long np, const 
         if (!gathered)
             hipLaunchKernelGGL((k_gather_rt<TX>), dim3(blocks_for(np * na, 256, 4096)), dim3(256),
                                0, s, na, d, np, X, S);
+    }
+    auto go = [&](auto dn, auto dl, auto sm) {
+        constexpr bool DN = decltype(dn)::value, DL = decltype(dl)::value;
+        constexpr bool SM = decltype(sm)::value && sizeof(TT) == 4;
+        hipLaunchKernelGGL((k_level_ks<TT, TX, DN, DL, SM>), g, b, 0, s, na, ldp, d, np, PT, S, X,
+                           DN ? Sn : nullptr, DN ? npn : 0L);
+    };
+    using T0 = std::false_type;
+    using T1 = std::true_type;
+    if (down) {
         if (delta)
-            hipLaunchKernelGGL((k_level_ks<TT, TX, true, true>), g, b, 0, s, na, ldp, d, np, PT, S,
-                               X, Sn, npn);
+            small ? go(T1{}, T1{}, T1{}) : go(T1{}, T1{}, T0{});
         else
-            hipLaunchKernelGGL((k_level_ks<TT, TX, true, false>), g, b, 0, s, na, ldp, d, np, PT,
-                               S, X, Sn, npn);
-    } else if (delta) {
-        hipLaunchKernelGGL((k_level_ks<TT, TX, false, true>), g, b, 0, s, na, ldp, d, np, PT, S, X,
-                           nullptr, 0L);
+            small ? go(T1{}, T0{}, T1{}) : go(T1{}, T0{}, T0{});
     } else {
-        hipLaunchKernelGGL((k_level_ks<TT, TX, false, false>), g, b, 0, s, na, ldp, d, np, PT, S,
-                           X, nullptr, 0L);
+        if (delta)
+            small ? go(T0{}, T1{}, T1{}) : go(T0{}, T1{}, T0{});
+        else
+            small ? go(T0{}, T0{}, T1{}) : go(T0{}, T0{}, T0{});
     }
     check_launch("level_ks");
 }
