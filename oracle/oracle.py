@@ -53,6 +53,12 @@ def _load():
         "or_simulate_mirror_exact": [_c_int, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p,
                                      _c_int, _c_int, _opt_f32, _opt_f32, _c_double, _c_double,
                                      _c_double, _f32p, _f64p],
+        "or_simulate_mirror_nsb": [_c_int, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p,
+                                   _c_int, _c_int, _opt_f32, _opt_f32, _c_double, _c_double,
+                                   _c_double, _f32p, _c_long, _f32p, _f32p],
+        "or_simulate_mirror_exact_nsb": [_c_int, _c_int, _c_int, _c_int, _f32p, _f32p, _f32p,
+                                         _f32p, _c_int, _c_int, _opt_f32, _opt_f32, _c_double,
+                                         _c_double, _c_double, _f32p, _c_long, _f64p],
         "or_simulate_f64": [_c_int, _c_int, _c_int, _c_int, _f64p, _f64p, _f64p, _f64p,
                             _c_int, _c_int, _f64p, _c_long, _c_double, _f64p, _f64p, _opt_f32],
         "or_cpu_rk4_odeint": [_c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p, _c_int, _c_int,
@@ -69,6 +75,8 @@ def _load():
     lib.or_simulate_mirror.restype = _c_long
     lib.or_simulate_f64.restype = _c_long
     lib.or_simulate_mirror_exact.restype = _c_long
+    lib.or_simulate_mirror_nsb.restype = _c_long
+    lib.or_simulate_mirror_exact_nsb.restype = _c_long
     lib.or_cpu_rk4_odeint.restype = _c_long
     lib.or_cpu_abm_pece.restype = _c_long
     lib.or_gauss_noise.restype = _c_double
@@ -233,8 +241,10 @@ def time_vector(solver, t0, h, total):
 
 
 # ------------------------------------------------------------- whole solvers
-def simulate_mirror(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None):
-    """FP32-mirror restatement of the reference's SimulateODE.
+def simulate_mirror(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None, nsb=0):
+    """FP32-mirror restatement of the reference's SimulateODE, in batches of
+    nsb elements (0: one batch; the input grid restarts per batch,
+    GPUFixedStepIterative.cpp:191-234).
 
     Returns (tVect, xVect[total, nOut])."""
     A = _f32(A)
@@ -251,17 +261,18 @@ def simulate_mirror(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None):
     xv = np.empty((total, nOut), np.float32)
     uc = _f32(ucoef) if ucoef is not None else None
     uh = _f32(Uhalf) if Uhalf is not None else None
-    got = lib().or_simulate_mirror(
+    got = lib().or_simulate_mirror_nsb(
         solver, n, m, p, A, _nz(B, np.float32), _nz(C, np.float32), _nz(D, np.float32), zC, zD,
         uc.ctypes.data if uc is not None else None, uh.ctypes.data if uh is not None else None,
-        t0, t1, h, _f32(x0), tv, xv)
+        t0, t1, h, _f32(x0), int(nsb), tv, xv)
     assert got == total
     return tv, xv
 
 
-def simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None):
+def simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=None, nsb=0):
     """The reference's algorithm in exact arithmetic on the reference's float
-    inputs (FP32-mirror M, Bbar, samples and start-up states; FP64 scan).
+    inputs (FP32-mirror M, Bbar, samples and start-up states; FP64 scan), in
+    batches of nsb elements (0: one batch; see simulate_mirror).
     Returns xVect[total, nOut] in FP64."""
     A = _f32(A)
     n = A.shape[0]
@@ -276,10 +287,10 @@ def simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=None, Uhalf=N
     y = np.empty((total, nOut))
     uc = _f32(ucoef) if ucoef is not None else None
     uh = _f32(Uhalf) if Uhalf is not None else None
-    got = lib().or_simulate_mirror_exact(
+    got = lib().or_simulate_mirror_exact_nsb(
         solver, n, m, p, A, _nz(B, np.float32), _nz(C, np.float32), _nz(D, np.float32), zC, zD,
         uc.ctypes.data if uc is not None else None, uh.ctypes.data if uh is not None else None,
-        t0, t1, h, _f32(x0), y)
+        t0, t1, h, _f32(x0), int(nsb), y)
     assert got == total
     return y
 

@@ -771,186 +771,264 @@ static void outputs_f64(int n, int m, int p, const double *C, const double *D, i
     }
 }
 
-/* The reference's SimulateODE in FP32-mirror arithmetic, single batch (what the
- * reference does whenever the horizon fits in device memory,
- * RungeKutta4thOrder.cpp:126-142).  The input is either the synthetic UFunc
- * family (ucoef, m x 6, evaluated on the reference's float time grids) or,
- * when ucoef is NULL, float samples Uhalf on the h/2 grid from tStart
+/* ------------------------------------------------------------------------- */
+/* batches of the reference's driver                                         */
+/* ------------------------------------------------------------------------- */
+
+/* The reference runs the scan in batches of NSB elements (a power of two; the
+ * last batch is padded up to the next power of two and its padded outputs are
+ * dropped, GPUFixedStepIterative.cpp:191-221, :443).  Batch b starts at
+ * t_b = tStart + (b NSB) h in double (:224) and samples its inputs on a float
+ * grid that restarts there:
+ *   RK4: u_b[s] = UFunc(float(t_b) + float(h/2) s), s = 0 .. 2 cnt
+ *        (RungeKutta4thOrder.cpp:255-257, Kernels/u.cl:8);
+ *   ABM: u_b[s] = UFunc(float(t_b - 3h) + float(h) s), s = 0 .. cnt + 3
+ *        (AdamsMoulton.cpp:426-428; tStart already shifted by 3h, :356).
+ * Element e = b NSB + i takes its leaf and output inputs from u_b at local
+ * index i.  The carried accumulator (:294-337) continues the recurrence
+ * across batches, so the state sequence is that of one scan; only the input
+ * samples depend on the batching.  Sampled inputs (Uhalf on the h/2 grid from
+ * tStart) are indexed globally and do not restart.
+ * nsb <= 0 or nsb >= N: one batch (what the reference does whenever the
+ * horizon fits in device memory, RungeKutta4thOrder.cpp:126-142). */
+static long batch_len(long N, long nsb) { return (nsb <= 0 || nsb >= N) ? N : nsb; }
+
+/* samples of the batch starting at element k0 with cnt real elements */
+static void batch_samples(int solver, int m, const float *ucoef, const float *Uhalf,
+                          double tStart, double h, long k0, long cnt, float *U)
+{
+    if (!m) return;
+    if (solver == 0) {
+        const long S = 2 * cnt + 1;
+        if (ucoef) {
+            float *t = malloc((size_t)S * 4);
+            or_time_grid_f32(tStart + (double)k0 * h, h / 2.0, 0, S, t);
+            or_ufunc_eval_f32(m, ucoef, S, t, U);
+            free(t);
+        } else memcpy(U, Uhalf + 2 * k0 * m, (size_t)S * m * 4);
+        return;
+    }
+    const long S = cnt + 4;   /* tStart: the shifted t_0 */
+    if (ucoef) {
+        float *t = malloc((size_t)S * 4);
+        or_time_grid_f32(tStart + (double)k0 * h - 3.0 * h, h, 0, S, t);
+        or_ufunc_eval_f32(m, ucoef, S, t, U);
+        free(t);
+    } else {
+        /* h/2-grid samples from the original tStart: u(t_{e}) of the shifted
+         * grid's row s is row 2 (k0 + s) */
+        for (long s = 0; s < S; ++s)
+            memcpy(U + s * m, Uhalf + 2 * (k0 + s) * m, (size_t)m * 4);
+    }
+}
+
+/* The reference's SimulateODE in FP32-mirror arithmetic with batches of nsb
+ * elements (see batch_len).  The input is either the synthetic UFunc family
+ * (ucoef, m x 6, evaluated on the reference's float time grids) or, when
+ * ucoef is NULL, float samples Uhalf on the h/2 grid from tStart
  * (Uhalf[s] = u(tStart + s h/2), 2*total+1 rows).
  * m = 0 stands for ZeroB (no inputs).  p = rows of C/D (ignored when both are
  * absent; the output is then the state, n wide).  xVect is time-major,
  * xVect[k*nOut + j] (GPUODESolver.cpp:36-42).  Returns the number of points. */
-OR_EXPORT long or_simulate_mirror(int solver, int n, int m, int p,
-                                  const float *A, const float *B, const float *C, const float *D,
-                                  int zC, int zD, const float *ucoef, const float *Uhalf,
-                                  double tStart, double tEnd, double h, const float *x0,
-                                  float *tVect, float *xVect)
+OR_EXPORT long or_simulate_mirror_nsb(int solver, int n, int m, int p,
+                                      const float *A, const float *B, const float *C, const float *D,
+                                      int zC, int zD, const float *ucoef, const float *Uhalf,
+                                      double tStart, double tEnd, double h, const float *x0,
+                                      long nsb, float *tVect, float *xVect)
 {
     long total = or_step_count(solver, tStart, tEnd, h);
     if (total < 1) return -1;
     int nOut = (zC && zD) ? n : p;
     or_time_vector(solver, tStart, h, total, tVect);
     if (solver == 0) {
-        long N = total;
+        const long N = total, NSB = batch_len(N, nsb);
         float *M = malloc((size_t)n * n * 4), *Bb = malloc((size_t)n * 3 * (m ? m : 1) * 4);
         or_rk4_build_mirror(n, m, A, B, h, M, Bb);
-        long S = 2 * N + 1;
-        float *U = malloc((size_t)S * (m ? m : 1) * 4);
-        if (m) {
-            if (ucoef) {
-                float *t = malloc((size_t)S * 4);
-                or_time_grid_f32(tStart, h / 2.0, 0, S, t);
-                or_ufunc_eval_f32(m, ucoef, S, t, U);
-                free(t);
-            } else memcpy(U, Uhalf, (size_t)S * m * 4);
-        }
+        float *U = malloc((size_t)(2 * NSB + 1) * (m ? m : 1) * 4);
         float *V = calloc((size_t)N * n, 4), *X = malloc((size_t)N * n * 4);
-        if (m) or_rk4_leaves_f32(n, m, Bb, N, U, V);
+        for (long k0 = 0; k0 < N; k0 += NSB) {
+            const long cnt = N - k0 < NSB ? N - k0 : NSB;
+            batch_samples(0, m, ucoef, Uhalf, tStart, h, k0, cnt, U);
+            if (m) or_rk4_leaves_f32(n, m, Bb, cnt, U, V + k0 * n);
+        }
         or_recurrence_f32(n, M, N, V, x0, X);
-        for (long e = 0; e < N; ++e)
-            outputs_f32(n, m, p, C, D, zC, zD, X + e * n, m ? U + 2 * e * m : NULL, xVect + e * nOut);
+        for (long k0 = 0; k0 < N; k0 += NSB) {
+            const long cnt = N - k0 < NSB ? N - k0 : NSB;
+            batch_samples(0, m, ucoef, Uhalf, tStart, h, k0, cnt, U);
+            for (long i = 0; i < cnt; ++i)
+                outputs_f32(n, m, p, C, D, zC, zD, X + (k0 + i) * n, m ? U + 2 * i * m : NULL,
+                            xVect + (k0 + i) * nOut);
+        }
         free(M); free(Bb); free(U); free(V); free(X);
         return total;
     }
     /* ABM */
-    long N = total - 3;
+    const long N = total - 3, NSB = batch_len(N, nsb);
     const int na = 4 * n;
     float *M = malloc((size_t)na * na * 4), *Bb = malloc((size_t)n * 5 * (m ? m : 1) * 4);
     or_abm_build_mirror(n, m, A, B, h, M, Bb);
     float Uh0[9 * 64];
     float *Uh = m <= 64 ? Uh0 : malloc((size_t)9 * m * 4);
-    double ts = tStart + 3.0 * h;
-    long S = N + 4;
-    float *U = malloc((size_t)S * (m ? m : 1) * 4);
+    const double ts = tStart + 3.0 * h;
     if (m) {
         if (ucoef) {
             float t9[9];
             or_time_grid_f32(tStart, h / 2.0, 0, 9, t9);          /* AdamsMoulton.cpp:64-65 */
             or_ufunc_eval_f32(m, ucoef, 9, t9, Uh);
-            float *t = malloc((size_t)S * 4);
-            or_time_grid_f32(ts - 3.0 * h, h, 0, S, t);            /* AdamsMoulton.cpp:426-427 */
-            or_ufunc_eval_f32(m, ucoef, S, t, U);
-            free(t);
-        } else {
-            memcpy(Uh, Uhalf, (size_t)9 * m * 4);
-            for (long s = 0; s < S; ++s) memcpy(U + s * m, Uhalf + 2 * s * m, (size_t)m * 4);
-        }
+        } else memcpy(Uh, Uhalf, (size_t)9 * m * 4);
     }
+    float *U = malloc((size_t)(NSB + 4) * (m ? m : 1) * 4);
     float *X0 = malloc((size_t)na * 4);
     or_abm_startup_f32(n, m, A, B, x0, Uh, h, X0);
-    float *V = calloc((size_t)N * na, 4);
-    if (m) or_abm_leaves_f32(n, m, Bb, N, U, V);
-    float *Vl = malloc((size_t)N * n * 4), *Xh = malloc((size_t)(N + 3) * n * 4);
-    for (long e = 0; e < N; ++e) memcpy(Vl + e * n, V + e * na + 3 * n, (size_t)n * 4);
+    float *V = calloc((size_t)NSB * na, 4);
+    float *Vl = calloc((size_t)N * n, 4), *Xh = malloc((size_t)(N + 3) * n * 4);
+    for (long k0 = 0; k0 < N; k0 += NSB) {
+        const long cnt = N - k0 < NSB ? N - k0 : NSB;
+        batch_samples(1, m, ucoef, Uhalf, ts, h, k0, cnt, U);
+        if (m) or_abm_leaves_f32(n, m, Bb, cnt, U, V);
+        for (long e = 0; m && e < cnt; ++e)
+            memcpy(Vl + (k0 + e) * n, V + e * na + 3 * n, (size_t)n * 4);
+    }
     abm_hist_f32(n, M + (size_t)3 * n * na, N, Vl, X0, Xh);
     /* first three outputs from the start-up states and u(t0), u(t0+h), u(t0+2h)
      * (GPUFixedStepIterative.cpp:150-173; AdamsMoulton.cpp:106-116) */
     for (int i = 0; i < 3 && i < total; ++i)
         outputs_f32(n, m, p, C, D, zC, zD, X0 + (size_t)i * n, m ? Uh + (size_t)2 * i * m : NULL,
                     xVect + (size_t)i * nOut);
-    /* scan outputs: state block 3 of element e (= x_{e+3}) and input sample e+3 */
-    for (long e = 0; e < N; ++e)
-        outputs_f32(n, m, p, C, D, zC, zD, Xh + (e + 3) * n, m ? U + (e + 3) * m : NULL,
-                    xVect + (3 + e) * nOut);
+    /* scan outputs: state block 3 of element e (= x_{e+3}) and input sample e+3
+     * of its batch */
+    for (long k0 = 0; k0 < N; k0 += NSB) {
+        const long cnt = N - k0 < NSB ? N - k0 : NSB;
+        batch_samples(1, m, ucoef, Uhalf, ts, h, k0, cnt, U);
+        for (long i = 0; i < cnt; ++i)
+            outputs_f32(n, m, p, C, D, zC, zD, Xh + (k0 + i + 3) * n, m ? U + (i + 3) * m : NULL,
+                        xVect + (3 + k0 + i) * nOut);
+    }
     if (Uh != Uh0) free(Uh);
     free(M); free(Bb); free(U); free(X0); free(V); free(Vl); free(Xh);
     return total;
 }
 
+OR_EXPORT long or_simulate_mirror(int solver, int n, int m, int p,
+                                  const float *A, const float *B, const float *C, const float *D,
+                                  int zC, int zD, const float *ucoef, const float *Uhalf,
+                                  double tStart, double tEnd, double h, const float *x0,
+                                  float *tVect, float *xVect)
+{
+    return or_simulate_mirror_nsb(solver, n, m, p, A, B, C, D, zC, zD, ucoef, Uhalf, tStart, tEnd,
+                                  h, x0, 0, tVect, xVect);
+}
+
 /* The reference's algorithm in exact arithmetic on the reference's float
- * inputs: M, Bbar, the input samples (float grid + float UFunc) and the ABM
- * start-up states are formed exactly as or_simulate_mirror forms them (FP32),
- * then promoted to double; leaves, the recurrence and the outputs run in FP64.
- * Any correct evaluation order of the reference's scan on those inputs (its
- * Blelloch tree, a sequential loop, the collapsed power-table scan) agrees
- * with this to O(log N) float roundings, which is what the FP32 parity bar
- * measures.  yOut is time-major FP64. */
+ * inputs: M, Bbar, the input samples (float grids + float UFunc, restarted
+ * per batch of nsb elements as in or_simulate_mirror_nsb) and the ABM
+ * start-up states are formed exactly as or_simulate_mirror_nsb forms them
+ * (FP32), then promoted to double; leaves, the recurrence and the outputs run
+ * in FP64.  Any correct evaluation order of the reference's scan on those
+ * inputs (its Blelloch tree, a sequential loop, the collapsed power-table
+ * scan) agrees with this to O(log N) float roundings, which is what the FP32
+ * parity bar measures.  yOut is time-major FP64. */
+OR_EXPORT long or_simulate_mirror_exact_nsb(int solver, int n, int m, int p,
+                                            const float *A, const float *B, const float *C,
+                                            const float *D, int zC, int zD, const float *ucoef,
+                                            const float *Uhalf, double tStart, double tEnd,
+                                            double h, const float *x0, long nsb, double *yOut)
+{
+    long total = or_step_count(solver, tStart, tEnd, h);
+    if (total < 1) return -1;
+    int nOut = (zC && zD) ? n : p;
+    const int mm = m ? m : 1;
+    double *Cd = malloc((size_t)(p > 0 ? p : 1) * n * 8), *Dd = malloc((size_t)(p > 0 ? p : 1) * mm * 8);
+    for (long i = 0; !zC && i < (long)p * n; ++i) Cd[i] = C[i];
+    for (long i = 0; !zD && i < (long)p * m; ++i) Dd[i] = D[i];
+    if (solver == 0) {
+        const long N = total, NSB = batch_len(N, nsb), S = 2 * NSB + 1;
+        float *M = malloc((size_t)n * n * 4), *Bb = malloc((size_t)n * 3 * mm * 4);
+        or_rk4_build_mirror(n, m, A, B, h, M, Bb);
+        double *Md = malloc((size_t)n * n * 8), *Bd = malloc((size_t)n * 3 * mm * 8);
+        double *xd = malloc((size_t)n * 8);
+        for (long i = 0; i < (long)n * n; ++i) Md[i] = M[i];
+        for (long i = 0; i < (long)n * 3 * m; ++i) Bd[i] = Bb[i];
+        for (int i = 0; i < n; ++i) xd[i] = x0[i];
+        float *U = malloc((size_t)S * mm * 4);
+        double *Ud = malloc((size_t)S * mm * 8);
+        double *V = calloc((size_t)N * n, 8), *X = malloc((size_t)N * n * 8);
+        for (long k0 = 0; m && k0 < N; k0 += NSB) {
+            const long cnt = N - k0 < NSB ? N - k0 : NSB;
+            batch_samples(0, m, ucoef, Uhalf, tStart, h, k0, cnt, U);
+            for (long i = 0; i < (2 * cnt + 1) * m; ++i) Ud[i] = U[i];
+            or_rk4_leaves_f64(n, m, Bd, cnt, Ud, V + k0 * n);
+        }
+        or_recurrence_f64(n, Md, N, V, xd, X);
+        for (long k0 = 0; k0 < N; k0 += NSB) {
+            const long cnt = N - k0 < NSB ? N - k0 : NSB;
+            batch_samples(0, m, ucoef, Uhalf, tStart, h, k0, cnt, U);
+            for (long i = 0; m && i < (2 * cnt + 1) * m; ++i) Ud[i] = U[i];
+            for (long i = 0; i < cnt; ++i)
+                outputs_f64(n, m, p, Cd, Dd, zC, zD, X + (k0 + i) * n, m ? Ud + 2 * i * m : NULL,
+                            yOut + (k0 + i) * nOut);
+        }
+        free(M); free(Bb); free(U); free(Md); free(Bd); free(Ud); free(xd); free(V); free(X);
+        free(Cd); free(Dd);
+        return total;
+    }
+    const long N = total - 3, NSB = batch_len(N, nsb), S = NSB + 4;
+    const int na = 4 * n;
+    float *M = malloc((size_t)na * na * 4), *Bb = malloc((size_t)n * 5 * mm * 4);
+    or_abm_build_mirror(n, m, A, B, h, M, Bb);
+    float *Uh = malloc((size_t)9 * mm * 4), *U = malloc((size_t)S * mm * 4);
+    const double ts = tStart + 3.0 * h;
+    if (m) {
+        if (ucoef) {
+            float t9[9];
+            or_time_grid_f32(tStart, h / 2.0, 0, 9, t9);
+            or_ufunc_eval_f32(m, ucoef, 9, t9, Uh);
+        } else memcpy(Uh, Uhalf, (size_t)9 * m * 4);
+    }
+    float *X0 = malloc((size_t)na * 4);
+    or_abm_startup_f32(n, m, A, B, x0, Uh, h, X0);
+    double *Mb = malloc((size_t)n * na * 8), *Bd = malloc((size_t)n * 5 * mm * 8);
+    double *Ud = malloc((size_t)S * mm * 8), *Uhd = malloc((size_t)9 * mm * 8);
+    double *X0d = malloc((size_t)na * 8);
+    for (long i = 0; i < (long)n * na; ++i) Mb[i] = M[(size_t)3 * n * na + i];
+    for (long i = 0; i < (long)n * 5 * m; ++i) Bd[i] = Bb[i];
+    for (long i = 0; i < 9L * m; ++i) Uhd[i] = Uh[i];
+    for (int i = 0; i < na; ++i) X0d[i] = X0[i];
+    double *V = calloc((size_t)NSB * na, 8);
+    double *Vl = calloc((size_t)N * n, 8), *Xh = malloc((size_t)(N + 3) * n * 8);
+    for (long k0 = 0; m && k0 < N; k0 += NSB) {
+        const long cnt = N - k0 < NSB ? N - k0 : NSB;
+        batch_samples(1, m, ucoef, Uhalf, ts, h, k0, cnt, U);
+        for (long i = 0; i < (cnt + 4) * m; ++i) Ud[i] = U[i];
+        or_abm_leaves_f64(n, m, Bd, cnt, Ud, V);
+        for (long e = 0; e < cnt; ++e) memcpy(Vl + (k0 + e) * n, V + e * na + 3 * n, (size_t)n * 8);
+    }
+    abm_hist_f64(n, Mb, N, Vl, X0d, Xh);
+    for (int i = 0; i < 3; ++i)
+        outputs_f64(n, m, p, Cd, Dd, zC, zD, X0d + (size_t)i * n, m ? Uhd + (size_t)2 * i * m : NULL,
+                    yOut + (size_t)i * nOut);
+    for (long k0 = 0; k0 < N; k0 += NSB) {
+        const long cnt = N - k0 < NSB ? N - k0 : NSB;
+        batch_samples(1, m, ucoef, Uhalf, ts, h, k0, cnt, U);
+        for (long i = 0; m && i < (cnt + 4) * m; ++i) Ud[i] = U[i];
+        for (long i = 0; i < cnt; ++i)
+            outputs_f64(n, m, p, Cd, Dd, zC, zD, Xh + (k0 + i + 3) * n, m ? Ud + (i + 3) * m : NULL,
+                        yOut + (3 + k0 + i) * nOut);
+    }
+    free(M); free(Bb); free(Uh); free(U); free(X0); free(Mb); free(Bd); free(Ud); free(Uhd);
+    free(X0d); free(V); free(Vl); free(Xh); free(Cd); free(Dd);
+    return total;
+}
+
 OR_EXPORT long or_simulate_mirror_exact(int solver, int n, int m, int p,
                                         const float *A, const float *B, const float *C, const float *D,
                                         int zC, int zD, const float *ucoef, const float *Uhalf,
                                         double tStart, double tEnd, double h, const float *x0,
                                         double *yOut)
 {
-    long total = or_step_count(solver, tStart, tEnd, h);
-    if (total < 1) return -1;
-    int nOut = (zC && zD) ? n : p;
-    double *Cd = malloc((size_t)(p > 0 ? p : 1) * n * 8), *Dd = malloc((size_t)(p > 0 ? p : 1) * (m ? m : 1) * 8);
-    for (long i = 0; !zC && i < (long)p * n; ++i) Cd[i] = C[i];
-    for (long i = 0; !zD && i < (long)p * m; ++i) Dd[i] = D[i];
-    if (solver == 0) {
-        long N = total, S = 2 * N + 1;
-        float *M = malloc((size_t)n * n * 4), *Bb = malloc((size_t)n * 3 * (m ? m : 1) * 4);
-        or_rk4_build_mirror(n, m, A, B, h, M, Bb);
-        float *U = malloc((size_t)S * (m ? m : 1) * 4);
-        if (m) {
-            if (ucoef) {
-                float *t = malloc((size_t)S * 4);
-                or_time_grid_f32(tStart, h / 2.0, 0, S, t);
-                or_ufunc_eval_f32(m, ucoef, S, t, U);
-                free(t);
-            } else memcpy(U, Uhalf, (size_t)S * m * 4);
-        }
-        double *Md = malloc((size_t)n * n * 8), *Bd = malloc((size_t)n * 3 * (m ? m : 1) * 8);
-        double *Ud = malloc((size_t)S * (m ? m : 1) * 8), *xd = malloc((size_t)n * 8);
-        for (long i = 0; i < (long)n * n; ++i) Md[i] = M[i];
-        for (long i = 0; i < (long)n * 3 * m; ++i) Bd[i] = Bb[i];
-        for (long i = 0; i < S * m; ++i) Ud[i] = U[i];
-        for (int i = 0; i < n; ++i) xd[i] = x0[i];
-        double *V = calloc((size_t)N * n, 8), *X = malloc((size_t)N * n * 8);
-        if (m) or_rk4_leaves_f64(n, m, Bd, N, Ud, V);
-        or_recurrence_f64(n, Md, N, V, xd, X);
-        for (long e = 0; e < N; ++e)
-            outputs_f64(n, m, p, Cd, Dd, zC, zD, X + e * n, m ? Ud + 2 * e * m : NULL, yOut + e * nOut);
-        free(M); free(Bb); free(U); free(Md); free(Bd); free(Ud); free(xd); free(V); free(X);
-        free(Cd); free(Dd);
-        return total;
-    }
-    long N = total - 3, S = N + 4;
-    const int na = 4 * n;
-    float *M = malloc((size_t)na * na * 4), *Bb = malloc((size_t)n * 5 * (m ? m : 1) * 4);
-    or_abm_build_mirror(n, m, A, B, h, M, Bb);
-    float *Uh = malloc((size_t)9 * (m ? m : 1) * 4), *U = malloc((size_t)S * (m ? m : 1) * 4);
-    double ts = tStart + 3.0 * h;
-    if (m) {
-        if (ucoef) {
-            float t9[9];
-            or_time_grid_f32(tStart, h / 2.0, 0, 9, t9);
-            or_ufunc_eval_f32(m, ucoef, 9, t9, Uh);
-            float *t = malloc((size_t)S * 4);
-            or_time_grid_f32(ts - 3.0 * h, h, 0, S, t);
-            or_ufunc_eval_f32(m, ucoef, S, t, U);
-            free(t);
-        } else {
-            memcpy(Uh, Uhalf, (size_t)9 * m * 4);
-            for (long s = 0; s < S; ++s) memcpy(U + s * m, Uhalf + 2 * s * m, (size_t)m * 4);
-        }
-    }
-    float *X0 = malloc((size_t)na * 4);
-    or_abm_startup_f32(n, m, A, B, x0, Uh, h, X0);
-    double *Mb = malloc((size_t)n * na * 8), *Bd = malloc((size_t)n * 5 * (m ? m : 1) * 8);
-    double *Ud = malloc((size_t)S * (m ? m : 1) * 8), *Uhd = malloc((size_t)9 * (m ? m : 1) * 8);
-    double *X0d = malloc((size_t)na * 8);
-    for (long i = 0; i < (long)n * na; ++i) Mb[i] = M[(size_t)3 * n * na + i];
-    for (long i = 0; i < (long)n * 5 * m; ++i) Bd[i] = Bb[i];
-    for (long i = 0; i < S * m; ++i) Ud[i] = U[i];
-    for (long i = 0; i < 9L * m; ++i) Uhd[i] = Uh[i];
-    for (int i = 0; i < na; ++i) X0d[i] = X0[i];
-    double *V = calloc((size_t)N * na, 8);
-    if (m) or_abm_leaves_f64(n, m, Bd, N, Ud, V);
-    double *Vl = malloc((size_t)N * n * 8), *Xh = malloc((size_t)(N + 3) * n * 8);
-    for (long e = 0; e < N; ++e) memcpy(Vl + e * n, V + e * na + 3 * n, (size_t)n * 8);
-    abm_hist_f64(n, Mb, N, Vl, X0d, Xh);
-    for (int i = 0; i < 3; ++i)
-        outputs_f64(n, m, p, Cd, Dd, zC, zD, X0d + (size_t)i * n, m ? Uhd + (size_t)2 * i * m : NULL,
-                    yOut + (size_t)i * nOut);
-    for (long e = 0; e < N; ++e)
-        outputs_f64(n, m, p, Cd, Dd, zC, zD, Xh + (e + 3) * n, m ? Ud + (e + 3) * m : NULL,
-                    yOut + (3 + e) * nOut);
-    free(M); free(Bb); free(Uh); free(U); free(X0); free(Mb); free(Bd); free(Ud); free(Uhd);
-    free(X0d); free(V); free(Vl); free(Xh); free(Cd); free(Dd);
-    return total;
+    return or_simulate_mirror_exact_nsb(solver, n, m, p, A, B, C, D, zC, zD, ucoef, Uhalf, tStart,
+                                        tEnd, h, x0, 0, yOut);
 }
-
 /* FP64 precision reference.  Uhalf: FP64 samples on the h/2 grid from tStart,
  * 2*total+1 rows; time t_s = tStart + s h/2 is the caller's.  Writes y (time-
  * major, FP64) and, when Xout is not NULL, the states (n per point). */

@@ -214,11 +214,11 @@ def set_precision(mode):
     lib().ParODE_SetPrecision(mode)
 
 
-VECTOR_LEVELS, VECTOR_GEMM, NO_FUSED_TOP = 1, 2, 4
+VECTOR_LEVELS, VECTOR_GEMM, NO_FUSED_TOP, NO_CHUNKS = 1, 2, 4, 8
 
 
 def set_level_kernel(variant):
-    """Bit mask (0 = automatic): VECTOR_LEVELS | VECTOR_GEMM | NO_FUSED_TOP."""
+    """Bit mask (0 = automatic): VECTOR_LEVELS | VECTOR_GEMM | NO_FUSED_TOP | NO_CHUNKS."""
     lib().ParODE_SetLevelKernel(int(variant))
 
 

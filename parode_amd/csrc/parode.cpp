@@ -22,9 +22,11 @@ long& max_batch_steps()
     static long v = 0;
     return v;
 }
+// One text per thread: the server's worker threads and the callers of the
+// C ABI set it concurrently, and each caller reads back its own.
 std::string& last_error()
 {
-    static std::string s;
+    static thread_local std::string s;
     return s;
 }
 int& level_kernel()

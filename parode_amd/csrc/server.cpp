@@ -47,6 +47,13 @@ Layout layout(const ServeRequest& r)
     return L;
 }
 
+// Throws unless [off, off + bytes) lies inside a block of blockBytes.
+void check_range(const char* what, size_t off, size_t bytes, size_t blockBytes)
+{
+    if (off > blockBytes || bytes > blockBytes - off)
+        throw std::logic_error(std::string("Server: ") + what + " outside its block");
+}
+
 }  // namespace
 
 struct Server::Job {
@@ -165,10 +172,18 @@ bool Server::admit(Job& j, int lane)
         return false;
     }
     hipStream_t s = lanes_[lane];
+    // every host copy and every transfer stays inside its own block
+    const size_t uBytes = (size_t)(2 * r.points + 1) * r.m * 8;
+    const size_t yBytes = (size_t)r.points * L.nOut * 4;
+    check_range("staged matrices", 0, j.mats.size() * 4, L.stBytes);
+    check_range("staged coefficients", L.coef, j.coef.size() * 8, L.stBytes);
+    if (r.samples && r.m) check_range("staged samples", L.U, uBytes, L.stBytes);
+    check_range("upload", 0, L.h2d, std::min(L.stBytes, L.ioBytes));
+    check_range("device outputs", L.Y, yBytes, L.ioBytes);
+    check_range("staged outputs", L.stY, yBytes, L.stBytes);
     std::memcpy(j.stage, j.mats.data(), j.mats.size() * 4);
     if (!j.coef.empty()) std::memcpy(j.stage + L.coef, j.coef.data(), j.coef.size() * 8);
-    if (r.samples && r.m)
-        std::memcpy(j.stage + L.U, r.samples, (size_t)(2 * r.points + 1) * r.m * 8);
+    if (r.samples && r.m) std::memcpy(j.stage + L.U, r.samples, uBytes);
     PARODE_HIP(hipMemcpyAsync(j.io, j.stage, L.h2d, hipMemcpyHostToDevice, s));
     if (!r.samples && r.m)
         k::sample_family(s, r.m, reinterpret_cast<const double*>(j.io + L.coef), r.t0, 0.5 * r.h,
@@ -178,8 +193,7 @@ bool Server::admit(Job& j, int lane)
     float* dY = reinterpret_cast<float*>(j.io + L.Y);
     j.core->solve(s, r.points, r.h, f(L.A), r.m ? f(L.B) : nullptr, r.zC ? nullptr : f(L.C),
                   r.zD ? nullptr : f(L.D), f(L.x0), j.io + L.U, true, dY);
-    PARODE_HIP(hipMemcpyAsync(j.stage + L.stY, dY, (size_t)r.points * L.nOut * 4,
-                              hipMemcpyDeviceToHost, s));
+    PARODE_HIP(hipMemcpyAsync(j.stage + L.stY, dY, yBytes, hipMemcpyDeviceToHost, s));
     PARODE_HIP(hipEventCreateWithFlags(&j.done, hipEventDisableTiming));
     PARODE_HIP(hipEventRecord(j.done, s));
     j.lane = lane;

@@ -212,10 +212,10 @@ def test_batched_horizon_matches_single_batch(solver, batch):
         api.set_max_batch_steps(0)
     assert e == 0 and y.shape == (1000, 2)
     # batches restart the float time grid at float(t_batch) (AdamsMoulton.cpp:426,
-    # RungeKutta4thOrder.cpp:255): compare with the FP64 oracle on the same system
+    # RungeKutta4thOrder.cpp:255): the oracle runs the same batches
     ye = orc.simulate_mirror_exact(solver, s.A, s.B, s.C, s.D, s.x0, 0.0, s.t_end, s.h,
-                                   ucoef=s.ucoef)
-    assert orc.max_rel_err(y, ye) <= 2 * PARITY
+                                   ucoef=s.ucoef, nsb=batch)
+    assert orc.max_rel_err(y, ye) <= PARITY
 
 
 def test_invalid_inputs():
@@ -250,7 +250,7 @@ def test_sampled_float_inputs_plain_parity(solver):
     assert orc.max_rel_err(y, ye) <= PARITY
 
 
-@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-9), (api.MIXED, 1e-5)])
+@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-9), (api.MIXED, 1e-6)])
 @pytest.mark.parametrize("solver", [0, 1])
 def test_precision_modes_against_fp64(solver, mode, bound):
     s = synth.make_system(solver, 16, 2, 2, -10.0, -0.05, 5e-4, 1 << 14, seed=11,
@@ -269,10 +269,10 @@ def test_precision_modes_against_fp64(solver, mode, bound):
     err = orc.max_rel_err(y, y64)
     print(f"mode {mode} solver {solver}: max rel err vs FP64 {err:.3e}")
     # FP64 mode: the only FP32 rounding is the output cast (y is fType)
-    assert err <= max(bound, 1e-7)
+    assert err <= (max(bound, 1e-7) if mode == api.FP64 else bound)
 
 
-@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-5)])
+@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-6)])
 @pytest.mark.parametrize("solver,n,m,steps,f32", [(1, 17, 1, 1000, False), (1, 40, 3, 3001, True),
                                                   (0, 5, 7, 999, False), (0, 70, 5, 2049, True),
                                                   (1, 64, 8, 4097, False),
@@ -300,7 +300,7 @@ def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
     assert err <= bound
 
 
-@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-5)])
+@pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-6)])
 @pytest.mark.parametrize("solver,n,p", [(0, 200, 64), (1, 130, 40), (0, 90, 24), (1, 70, 17),
                                         (0, 300, 5)])
 def test_wide_state_outputs_against_fp64(solver, n, p, mode, bound):

@@ -318,3 +318,130 @@ def test_mirror_exact_is_fp64_recurrence_on_mirror_inputs():
         v = Bb[:, :m] @ U64[2 * k] + Bb[:, m:2 * m] @ U64[2 * k + 1] + Bb[:, 2 * m:] @ U64[2 * k + 2]
         x = M @ x + v
     np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------- batches (NSB)
+def _fmaf(a, b, c):
+    """Correctly rounded float32 fma(a, b, c) from the exact rational value."""
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    x = np.float32(float(exact))
+    cands = [x, np.nextafter(x, np.float32(np.inf)), np.nextafter(x, np.float32(-np.inf))]
+    best = min(cands, key=lambda v: (abs(Fraction(float(v)) - exact),
+                                     int(np.float32(v).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+def _batch_grid(t_first, dt, count):
+    """The reference's float grid fType(t_b) + fType(dt) * s (Kernels/u.cl:8)."""
+    tf, df = np.float32(t_first), np.float32(dt)
+    return np.array([_fmaf(df, np.float32(s), tf) for s in range(count)], np.float32)
+
+
+def _batch_system(solver, seed):
+    rng = np.random.default_rng(seed)
+    n, m, p = 3, 2, 2
+    A = (rng.standard_normal((n, n)) * 0.3 - 1.5 * np.eye(n)).astype(np.float32)
+    B = rng.standard_normal((n, m)).astype(np.float32)
+    C = rng.standard_normal((p, n)).astype(np.float32)
+    D = rng.standard_normal((p, m)).astype(np.float32)
+    x0 = rng.standard_normal(n).astype(np.float32)
+    coef = np.zeros((m, 6), np.float32)
+    coef[:, 0] = 1.0
+    coef[:, 4] = rng.uniform(1.0, 20.0, m)
+    coef[:, 5] = rng.uniform(0.0, 6.28, m)
+    return A, B, C, D, x0, coef
+
+
+@pytest.mark.parametrize("nsb", [4, 8])
+def test_rk4_batches_restart_the_float_grid(nsb):
+    """Brute force of the reference's batch loop for RK4 (GPUFixedStepIterative
+    .cpp:191-234, RungeKutta4thOrder.cpp:255-257): batch b covers elements
+    [b NSB, b NSB + cnt); its samples are UFunc(float(t_b) + float(h/2) s) with
+    t_b = tStart + (b NSB) h in double; each element's leaf and output read its
+    own batch's samples, and the recurrence continues across batches."""
+    A, B, C, D, x0, coef = _batch_system(0, 31)
+    t0, h, N = 7.3, 0.013, 21
+    t1 = t0 + (N - 1) * h + 1e-4 * h
+    y = orc.simulate_mirror_exact(0, A, B, C, D, x0, t0, t1, h, ucoef=coef, nsb=nsb)
+    M, Bb = orc.rk4_build_mirror(A, B, h)
+    M, Bb = M.astype(np.float64), Bb.astype(np.float64)
+    m = B.shape[1]
+    x = x0.astype(np.float64)
+    ref = np.empty((N, C.shape[0]))
+    for k0 in range(0, N, nsb):
+        cnt = min(nsb, N - k0)
+        tb = t0 + float(k0) * h
+        U = orc.ufunc_eval_f32(coef, _batch_grid(tb, h / 2.0, 2 * cnt + 1)).astype(np.float64)
+        for i in range(cnt):
+            ref[k0 + i] = C.astype(np.float64) @ x + D.astype(np.float64) @ U[2 * i]
+            v = (Bb[:, :m] @ U[2 * i] + Bb[:, m:2 * m] @ U[2 * i + 1]
+                 + Bb[:, 2 * m:] @ U[2 * i + 2])
+            x = M @ x + v
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("nsb", [4, 8])
+def test_abm_batches_restart_the_float_grid(nsb):
+    """The same for ABM (AdamsMoulton.cpp:356, :426-428): tStart is shifted by
+    3h, batch b samples UFunc(float(t_b - 3h) + float(h) s), s = 0 .. cnt + 3,
+    element i of the batch reads rows i .. i + 4 for its leaf and row i + 3 for
+    its output; the first three outputs come from the start-up states and the
+    h/2-grid samples 0, 2, 4 (AdamsMoulton.cpp:106-116)."""
+    A, B, C, D, x0, coef = _batch_system(1, 32)
+    t0, h, total = 3.1, 0.011, 24
+    t1 = t0 + (total - 1) * h + 1e-4 * h
+    y = orc.simulate_mirror_exact(1, A, B, C, D, x0, t0, t1, h, ucoef=coef, nsb=nsb)
+    n, m = A.shape[0], B.shape[1]
+    M, Bb = orc.abm_build_mirror(A, B, h)
+    M, Bb = M.astype(np.float64), Bb.astype(np.float64)
+    Uh = orc.ufunc_eval_f32(coef, _batch_grid(t0, h / 2.0, 9))
+    X = orc.abm_startup_f32(A, B, x0, Uh, h).astype(np.float64)
+    C64, D64 = C.astype(np.float64), D.astype(np.float64)
+    ref = np.empty((total, C.shape[0]))
+    for i in range(3):
+        ref[i] = C64 @ X[i * n:(i + 1) * n] + D64 @ Uh[2 * i].astype(np.float64)
+    ts, N = t0 + 3 * h, total - 3
+    for k0 in range(0, N, nsb):
+        cnt = min(nsb, N - k0)
+        tb = ts + float(k0) * h
+        U = orc.ufunc_eval_f32(coef, _batch_grid(tb - 3.0 * h, h, cnt + 4)).astype(np.float64)
+        for i in range(cnt):
+            ref[3 + k0 + i] = C64 @ X[3 * n:] + D64 @ U[i + 3]
+            v = np.zeros(4 * n)
+            v[3 * n:] = Bb @ U[i:i + 5].reshape(-1)
+            X = M @ X + v
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_sampled_inputs_do_not_depend_on_batches(solver):
+    """Sampled inputs are indexed over the whole horizon, so splitting it into
+    batches changes nothing: bitwise the single-batch result, in both modes."""
+    A, B, C, D, x0, _ = _batch_system(solver, 33)
+    h, total = 0.02, 37
+    U = np.random.default_rng(3).standard_normal((2 * total + 1, 2)).astype(np.float32)
+    t1 = (total - 1) * h + 1e-4 * h
+    one = orc.simulate_mirror_exact(solver, A, B, C, D, x0, 0.0, t1, h, Uhalf=U)
+    cut = orc.simulate_mirror_exact(solver, A, B, C, D, x0, 0.0, t1, h, Uhalf=U, nsb=8)
+    np.testing.assert_array_equal(one, cut)
+    _, one32 = orc.simulate_mirror(solver, A, B, C, D, x0, 0.0, t1, h, Uhalf=U)
+    _, cut32 = orc.simulate_mirror(solver, A, B, C, D, x0, 0.0, t1, h, Uhalf=U, nsb=4)
+    np.testing.assert_array_equal(one32, cut32)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_batch_grid_restart_is_visible_and_small(solver):
+    """Away from t = 0 the restarted grid rounds t differently from the
+    single-batch grid: the batched result differs from the single-batch one
+    (the first batch excepted) by the float time rounding only.  At t ~ 3
+    (ulp 2.4e-7, input frequencies up to 20) that is a few 1e-6; it grows with
+    t (1.5e-5 at t ~ 10, 1.2e-4 at t ~ 100), which is why a batched solve is
+    compared with the batched oracle, not the single-batch one."""
+    A, B, C, D, x0, coef = _batch_system(solver, 34)
+    t0, h, total = 3.0, 1e-3, 300
+    t1 = t0 + (total - 1) * h + 1e-4 * h
+    one = orc.simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=coef)
+    cut = orc.simulate_mirror_exact(solver, A, B, C, D, x0, t0, t1, h, ucoef=coef, nsb=64)
+    d = orc.max_rel_err(cut, one)
+    assert 0.0 < d < 1e-5
+    np.testing.assert_array_equal(cut[:64 + (3 if solver else 0)], one[:64 + (3 if solver else 0)])
