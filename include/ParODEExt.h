@@ -105,6 +105,26 @@ int ParODE_PlanDestroy(void* plan);
 /* Device memory the plan for this shape would hold, without allocating. */
 long ParODE_PlanEstimateBytes(int solver, int nStates, int nInputs, long points, int precision);
 
+/* ---- a solve that allocates no device memory ----
+ * No counterpart in the reference.  For callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The same solve as ParODE_PlanCreate + ParODE_PlanExecute (same
+ * kernels, same results), with the same arguments, but its scan buffers come
+ * from a static device array of ParODE_NoAllocScratchBytes() bytes (128 MiB)
+ * that each GPU holds until the process ends; a shape whose buffers
+ * (ParODE_PlanEstimateBytes, outputs included) exceed it returns
+ * ParODE_NotEnoughResources with the sizes in ParODE_GetLastErrorMessage.
+ * Calls no hipMalloc or hipFree.  Every call on one GPU shares that GPU's
+ * array: calls queued on one stream are safe, calls that overlap on one GPU
+ * (other streams, other threads) race. */
+int ParODE_SolveNoAlloc(int solver, int nStates, int nInputs, int nOutputs,
+						int zeroC, int zeroD, long points, double tStep,
+						int precision, int samplesDouble,
+						const float* dA, const float* dB, const float* dC,
+						const float* dD, const float* dx0, const void* dSamples,
+						float* dY, void* stream);
+long ParODE_NoAllocScratchBytes(void);
+
 /* ---- serving: continuous batching of independent solves on one GPU ----
  * For request loops with many systems of different shapes and horizons.  A
  * server owns a device arena (deviceBytes), a pinned staging area

@@ -25,7 +25,7 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_ScheduleClear", "ParODE_PlanChunkLog",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
-               "ParODE_ServerDestroy"]
+               "ParODE_ServerDestroy", "ParODE_SolveNoAlloc", "ParODE_NoAllocScratchBytes"]
 
 RK4, ABM = 0, 1
 PLAIN32, MIXED, FP64 = 0, 1, 2
@@ -107,6 +107,10 @@ def lib():
                                                ctypes.c_long, ctypes.c_int]
         L.ParODE_PlanEstimateBytes.restype = ctypes.c_long
         _l, _i32, _d = ctypes.c_long, ctypes.c_int, ctypes.c_double
+        L.ParODE_SolveNoAlloc.argtypes = ([_i32] * 6 + [_l, _d, _i32, _i32] + [_vp] * 8)
+        L.ParODE_SolveNoAlloc.restype = _i32
+        L.ParODE_NoAllocScratchBytes.argtypes = []
+        L.ParODE_NoAllocScratchBytes.restype = _l
         L.ParODE_ServerCreate.argtypes = [_i32, _l, _l, _i32, ctypes.POINTER(_vp)]
         L.ParODE_ServerCreate.restype = _i32
         L.ParODE_ServerSubmit.argtypes = ([_vp] + [_i32] * 6 + [_l, _d, _d] + [_fp] * 5 +
@@ -433,6 +437,20 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def solve_no_alloc(solver, n, m, p, zero_c, zero_d, points, h, precision, samples_double,
+                   dA, dB, dC, dD, dx0, dU, dY, stream=0):
+    """ParODE_SolveNoAlloc: the plan's solve with its scan buffers in the
+    library's static device scratch array (no device allocation).  Device
+    pointers as for Plan.execute.  Returns the status (0 = OK)."""
+    return lib().ParODE_SolveNoAlloc(solver, n, m, p, int(zero_c), int(zero_d), points, h,
+                                     precision, int(samples_double), dA, dB, dC, dD, dx0, dU, dY,
+                                     stream)
+
+
+def no_alloc_scratch_bytes():
+    return lib().ParODE_NoAllocScratchBytes()
 
 
 def schedule_clear():

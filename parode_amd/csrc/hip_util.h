@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <iterator>
 #include <map>
 #include <mutex>
@@ -31,23 +32,35 @@ inline void hip_check(hipError_t e, const char* what)
 class Arena {
 public:
     static constexpr size_t kAlign = 256;
-    enum Kind { kDevice, kPinnedHost };
+    enum Kind { kDevice, kPinnedHost, kBorrowed };
     Arena(size_t bytes, Kind kind) : kind_(kind)
     {
         cap_ = round(bytes);
         if (cap_ == 0) return;
         if (kind == kDevice)
             PARODE_HIP(hipMalloc(reinterpret_cast<void**>(&base_), cap_));
-        else
+        else if (kind == kPinnedHost)
             PARODE_HIP(hipHostMalloc(reinterpret_cast<void**>(&base_), cap_, hipHostMallocDefault));
+        else
+            throw std::invalid_argument("Arena: borrowed memory needs a base address");
         free_[0] = cap_;
+    }
+    // Carves blocks out of memory the caller owns (256-byte aligned base);
+    // allocates and frees no device memory.
+    Arena(void* base, size_t bytes) : kind_(kBorrowed)
+    {
+        if (reinterpret_cast<uintptr_t>(base) % kAlign)
+            throw std::invalid_argument("Arena: borrowed base not 256-byte aligned");
+        base_ = static_cast<char*>(base);
+        cap_ = bytes / kAlign * kAlign;
+        if (cap_) free_[0] = cap_;
     }
     ~Arena()
     {
         if (!base_) return;
         if (kind_ == kDevice)
             (void)hipFree(base_);
-        else
+        else if (kind_ == kPinnedHost)
             (void)hipHostFree(base_);
     }
     Arena(const Arena&) = delete;
