@@ -125,14 +125,46 @@ int ParODE_SolveNoAlloc(int solver, int nStates, int nInputs, int nOutputs,
 						float* dY, void* stream);
 long ParODE_NoAllocScratchBytes(void);
 
+/* ---- grouped solves: many systems, each stage of the scan one launch ----
+ * For batch callers with many independent systems of uneven state size and
+ * horizon (and the engine of the server below).  Mixed precision.  Every
+ * system's arrays live on the device: A n*n, B n*m, C p*n, D p*m, x0 n
+ * (float); either dCoef[i] (m x 6 sinusoid-family coefficients, see the
+ * server) with dU[i] receiving the FP64 samples, or dCoef NULL / dCoef[i]
+ * NULL and dU[i] holding (2*points+1)*m FP64 samples on the h/2 grid; dY[i]
+ * points*p outputs.  dC, dD, dCoef, tStart, sigma, seed may be NULL (absent,
+ * zeros).  ParODE_GroupEstimateBytes gives -1 for a shape outside the grouped
+ * kernels.  The group owns its scan buffers; executing it launches every
+ * stage for all systems on `stream` without host synchronisation.
+ * ExecuteTimed waits and writes the GPU time of each of ParODE_GroupStages()
+ * stages (names from ParODE_GroupStageName). */
+int ParODE_GroupCreate(int count, const int* solver, const int* nStates, const int* nInputs,
+					   const int* nOutputs, const long* points, const double* tStart,
+					   const double* tStep, const float* const* dA, const float* const* dB,
+					   const float* const* dC, const float* const* dD, const float* const* dx0,
+					   const double* const* dCoef, const double* sigma,
+					   const unsigned long long* seed, double* const* dU, float* const* dY,
+					   void** group);
+int ParODE_GroupExecute(void* group, void* stream);
+int ParODE_GroupExecuteTimed(void* group, void* stream, float* stageMs);
+int ParODE_GroupStages(void);
+const char* ParODE_GroupStageName(int stage);
+int ParODE_GroupLaunches(void* group);
+long ParODE_GroupFootprint(void* group);
+long ParODE_GroupEstimateBytes(int solver, int nStates, int nInputs, int nOutputs, long points);
+int ParODE_GroupDestroy(void* group);
+
 /* ---- serving: continuous batching of independent solves on one GPU ----
  * For request loops with many systems of different shapes and horizons.  A
  * server owns a device arena (deviceBytes), a pinned staging area
  * (pinnedBytes), `lanes` HIP streams (1..16) and a worker thread.  Submitted
  * requests are admitted in arrival order whenever a lane has room and the
- * arena holds the request's footprint (ParODE_ServerFootprint); an admitted
- * request's whole solve runs on its lane with no host synchronisation, so
- * requests overlap on the device and arrivals are admitted while others run.
+ * arena holds the request's footprint (ParODE_ServerFootprint).  With mixed
+ * precision, the requests waiting at the head of the queue are admitted
+ * together as one group whose scans run as grouped launches (each stage of
+ * the scan one launch for all of them); a group's work runs on its lane with
+ * no host synchronisation, so groups overlap on the device and arrivals are
+ * admitted while others run.
  * The server copies A, B, C, D, x0 and coef at submission; samples (host,
  * (2*points+1) x nInputs doubles on the h/2 grid from tStart) and Y (host,
  * points x nOut floats) must stay valid until the ticket completes.  With
@@ -148,6 +180,15 @@ int ParODE_ServerSubmit(void* server, int solver, int nStates, int nInputs, int 
 						const float* x0, const double* coef, double sigma,
 						unsigned long long seed, const double* samples, float* Y,
 						long* ticket);
+/* As ParODE_ServerSubmit, with the outputs written in place to dY, device
+ * memory of points x nOut floats that stays valid until the ticket
+ * completes: for serving loops whose consumers read the outputs on the GPU. */
+int ParODE_ServerSubmitDevice(void* server, int solver, int nStates, int nInputs, int nOutputs,
+							  int zeroC, int zeroD, long points, double tStart, double tStep,
+							  const float* A, const float* B, const float* C, const float* D,
+							  const float* x0, const double* coef, double sigma,
+							  unsigned long long seed, const double* samples, float* dY,
+							  long* ticket);
 /* Blocks until the ticket completes; returns its status.  timesMs (may be
  * NULL) receives [submission -> completion, submission -> admission]. */
 int ParODE_ServerWait(void* server, long ticket, double* timesMs);
@@ -155,6 +196,9 @@ int ParODE_ServerDrain(void* server);
 /* stats[8]: submitted, completed, failed, running, queued, peak running,
  * arena bytes, peak arena bytes in use. */
 int ParODE_ServerStats(void* server, long* stats);
+/* stats[3]: groups admitted (one lane's work each), requests that ran in
+ * grouped launches, the largest group. */
+int ParODE_ServerGroupStats(void* server, long* stats);
 long ParODE_ServerFootprint(int solver, int nStates, int nInputs, int nOutputs, long points,
 							int precision);
 int ParODE_ServerDestroy(void* server);

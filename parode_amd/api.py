@@ -25,7 +25,11 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_ScheduleClear", "ParODE_PlanChunkLog",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
-               "ParODE_ServerDestroy", "ParODE_SolveNoAlloc", "ParODE_NoAllocScratchBytes"]
+               "ParODE_ServerDestroy", "ParODE_SolveNoAlloc", "ParODE_NoAllocScratchBytes",
+               "ParODE_ServerSubmitDevice", "ParODE_ServerGroupStats", "ParODE_GroupCreate",
+               "ParODE_GroupExecute", "ParODE_GroupExecuteTimed", "ParODE_GroupStages",
+               "ParODE_GroupLaunches", "ParODE_GroupFootprint", "ParODE_GroupEstimateBytes",
+               "ParODE_GroupDestroy"]
 
 RK4, ABM = 0, 1
 PLAIN32, MIXED, FP64 = 0, 1, 2
@@ -117,6 +121,33 @@ def lib():
                                           [_dp, _d, ctypes.c_ulonglong, _dp, _fp,
                                            ctypes.POINTER(_l)])
         L.ParODE_ServerSubmit.restype = _i32
+        L.ParODE_ServerSubmitDevice.argtypes = ([_vp] + [_i32] * 6 + [_l, _d, _d] + [_fp] * 5 +
+                                                [_dp, _d, ctypes.c_ulonglong, _dp, _vp,
+                                                 ctypes.POINTER(_l)])
+        L.ParODE_ServerSubmitDevice.restype = _i32
+        L.ParODE_ServerGroupStats.argtypes = [_vp, ctypes.POINTER(_l)]
+        L.ParODE_ServerGroupStats.restype = _i32
+        _pp = ctypes.POINTER(_vp)
+        L.ParODE_GroupCreate.argtypes = ([_i32] + [_ip] * 4 + [ctypes.POINTER(_l), _dp, _dp] +
+                                         [_pp] * 6 + [_dp, ctypes.POINTER(ctypes.c_ulonglong)] +
+                                         [_pp] * 2 + [_pp])
+        L.ParODE_GroupCreate.restype = _i32
+        L.ParODE_GroupExecute.argtypes = [_vp, _vp]
+        L.ParODE_GroupExecute.restype = _i32
+        L.ParODE_GroupExecuteTimed.argtypes = [_vp, _vp, _fp]
+        L.ParODE_GroupExecuteTimed.restype = _i32
+        L.ParODE_GroupStages.argtypes = []
+        L.ParODE_GroupStages.restype = _i32
+        L.ParODE_GroupStageName.argtypes = [_i32]
+        L.ParODE_GroupStageName.restype = ctypes.c_char_p
+        L.ParODE_GroupLaunches.argtypes = [_vp]
+        L.ParODE_GroupLaunches.restype = _i32
+        L.ParODE_GroupFootprint.argtypes = [_vp]
+        L.ParODE_GroupFootprint.restype = _l
+        L.ParODE_GroupEstimateBytes.argtypes = [_i32] * 4 + [_l]
+        L.ParODE_GroupEstimateBytes.restype = _l
+        L.ParODE_GroupDestroy.argtypes = [_vp]
+        L.ParODE_GroupDestroy.restype = _i32
         L.ParODE_ServerWait.argtypes = [_vp, _l, _dp]
         L.ParODE_ServerWait.restype = _i32
         L.ParODE_ServerDrain.argtypes = [_vp]
@@ -408,6 +439,26 @@ class Server:
         self._keep[t.value] = (sm, Y)
         return t.value, Y
 
+    def submit_device(self, solver, A, B, C, D, x0, points, h, dY, t0=0.0, coef=None, sigma=0.0,
+                      seed=0, samples=None):
+        """Queues one solve whose outputs go to device memory dY (an address of
+        points x nOut floats, e.g. a torch tensor's data_ptr()); returns the ticket."""
+        A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
+        cf = None if coef is None else np.ascontiguousarray(coef, np.float64)
+        sm = None if samples is None else np.ascontiguousarray(samples, np.float64)
+        t = ctypes.c_long(0)
+        r = lib().ParODE_ServerSubmitDevice(
+            self.handle, solver, n, m, p, zC, zD, points, t0, h, _fptr(A), _fptr(Bf), _fptr(Cf),
+            _fptr(Df), _fptr(x0f),
+            None if cf is None else cf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            sigma, seed,
+            None if sm is None else sm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            dY, ctypes.byref(t))
+        if r != 0:
+            raise ParODEError(r, "ParODE_ServerSubmitDevice")
+        self._keep[t.value] = (sm, None)
+        return t.value
+
     def wait(self, ticket):
         ms = (ctypes.c_double * 2)()
         st = lib().ParODE_ServerWait(self.handle, ticket, ms)
@@ -421,6 +472,12 @@ class Server:
         v = (ctypes.c_long * 8)()
         lib().ParODE_ServerStats(self.handle, v)
         return dict(zip(self.STATS, list(v)))
+
+    def group_stats(self):
+        """groups admitted, requests run in grouped launches, the largest group"""
+        v = (ctypes.c_long * 3)()
+        lib().ParODE_ServerGroupStats(self.handle, v)
+        return dict(zip(("groups", "grouped_requests", "largest_group"), list(v)))
 
     @staticmethod
     def footprint(solver, n, m, p, points, precision=MIXED):
@@ -451,6 +508,68 @@ def solve_no_alloc(solver, n, m, p, zero_c, zero_d, points, h, precision, sample
 
 def no_alloc_scratch_bytes():
     return lib().ParODE_NoAllocScratchBytes()
+
+
+class Group:
+    """A device-resident grouped solve (ParODE_Group*): many systems, each stage
+    of the scan one launch for all of them, mixed precision.  systems: list of
+    dicts with keys method, n, m, p, points, h, t0 (optional), and device
+    addresses A, B, C, D, x0, U, Y (ints), plus coef (device address or None:
+    U then holds FP64 samples), sigma, seed (optional)."""
+
+    def __init__(self, systems):
+        k = len(systems)
+        ints = lambda key: (ctypes.c_int * k)(*[int(s[key]) for s in systems])
+        ptrs = lambda key: (_vp * k)(*[s.get(key) or None for s in systems])
+        self.handle = ctypes.c_void_p()
+        r = lib().ParODE_GroupCreate(
+            k, ints("method"), ints("n"), ints("m"), ints("p"),
+            (ctypes.c_long * k)(*[int(s["points"]) for s in systems]),
+            (ctypes.c_double * k)(*[float(s.get("t0", 0.0)) for s in systems]),
+            (ctypes.c_double * k)(*[float(s["h"]) for s in systems]),
+            ptrs("A"), ptrs("B"), ptrs("C"), ptrs("D"), ptrs("x0"), ptrs("coef"),
+            (ctypes.c_double * k)(*[float(s.get("sigma", 0.0)) for s in systems]),
+            (ctypes.c_ulonglong * k)(*[int(s.get("seed", 0)) for s in systems]),
+            ptrs("U"), ptrs("Y"), ctypes.byref(self.handle))
+        if r != 0:
+            raise ParODEError(r, "ParODE_GroupCreate")
+
+    def execute(self, stream=0):
+        r = lib().ParODE_GroupExecute(self.handle, stream)
+        if r != 0:
+            raise ParODEError(r, "ParODE_GroupExecute")
+
+    def execute_timed(self, stream=0):
+        """Execute, wait, and return {stage: GPU ms} measured with hipEvents."""
+        k = lib().ParODE_GroupStages()
+        ms = (ctypes.c_float * k)()
+        r = lib().ParODE_GroupExecuteTimed(self.handle, stream, ms)
+        if r != 0:
+            raise ParODEError(r, "ParODE_GroupExecuteTimed")
+        return {lib().ParODE_GroupStageName(i).decode(): ms[i] for i in range(k)}
+
+    @property
+    def launches(self):
+        return lib().ParODE_GroupLaunches(self.handle)
+
+    @property
+    def footprint(self):
+        return lib().ParODE_GroupFootprint(self.handle)
+
+    @staticmethod
+    def estimate_bytes(method, n, m, p, points):
+        return lib().ParODE_GroupEstimateBytes(method, n, m, p, points)
+
+    def close(self):
+        if self.handle:
+            lib().ParODE_GroupDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def schedule_clear():

@@ -10,11 +10,31 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <map>
+#include <utility>
+#include <vector>
 
 namespace parode {
 namespace k {
 
 // ---------------------------------------------------------------- step maps
+// Scalar factors of the RK4 map (RungeKutta4thOrder.cpp:48-57) and of the ABM
+// companion blocks (AdamsMoulton.cpp:258-269), evaluated on the host left to
+// right; the chunk recurrences' factors of an FP64 ABM map (blocks oldest
+// first: x_{k-3} (A^2 only), x_{k-2}, x_{k-1}, x_k).
+struct Rk4Coef {
+    double h, c2, c3, c4, s1, s2, s3, t0, t1, t2, u0;
+};
+struct AbmCoef {
+    double a0A, a0A2, a1A, a1A2, a2A, a2A2, a3A2, bm1, b0B, b0AB, b1B, b1AB, b2B, b2AB, b3AB;
+};
+struct AbmFactors {
+    float a[4], b[4];
+};
+Rk4Coef rk4_coefficients(double h);
+AbmCoef abm_coefficients(double h);
+AbmFactors abm_chunk_factors(double h);
+
 // Z = X * Y.  mirror = true: float products, sequential-k float accumulation,
 // no fused multiply-add (the rounding of uBLAS prod() on matrix<float>).
 void gemm_seq_f32(hipStream_t s, int r, int kk, int c, const float* X, const float* Y, float* Z);
@@ -200,6 +220,86 @@ void sample_family(hipStream_t s, int m, const double* coef, double t0, double d
 // Copies a typed vector into FP64.
 template <typename T>
 void to_f64(hipStream_t s, long count, const T* in, double* out);
+
+
+// ---------------------------------------------------------- grouped launches
+// One launch runs the same kernel for many systems (the served path): each
+// system owns a run of the grid's work-groups and an argument record, and
+// its work-groups run exactly the single-system body on the system's own
+// buffers.  A phase collects the launches of one stage of the scan for all
+// systems, keyed by kernel (and pass or sweep direction); the planner packs
+// every launch's records and block table into one device buffer and runs the
+// launches in key order.
+namespace g {
+enum Kind {
+    kPrep, kGemm16, kGemmKs2, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
+    kSample, kStartup, kStartOut, kLeavesPlain, kLeavesFac, kLeavesSimt,
+    kChunksFac16, kChunksFac32, kChunksFac64, kChunksFac128,
+    kChunksTwo160, kChunksTwo192, kChunksTwo224, kChunksTwo256,
+    kDense16, kDense32, kDense64, kDense128, kDense256,
+    kLevel32, kLevel32Delta, kLevel64, kGather, kTopMfma, kTopSimt, kRoot,
+    kOutputsLanes, kOutputsKc8, kOutputsKc16, kOutputsKc24, kOutputsKc32
+};
+struct Launch {
+    int kind = -1;
+    bool flag = false;                  // pass 2 of the chunks, or the down-sweep
+    size_t argSize = 0;                 // bytes of one argument record
+    std::vector<unsigned char> args;    // nsys records
+    std::vector<unsigned> start{0};     // nsys + 1: first block of each system
+    size_t lds = 0;                     // dynamic LDS bytes (the largest system's)
+    int nsys() const { return (int)start.size() - 1; }
+    unsigned blocks() const { return start.back(); }
+};
+using Phase = std::map<std::pair<int, int>, Launch>;
+
+// Each add_* appends one system's launch of a kernel to a phase (its kind and
+// launch shape chosen as the single-system launcher would choose them).
+void add_cast(Phase& ph, long count, const float* in, double* out);
+void add_fill(Phase& ph, long count, double v, double* out);
+// out[r][c] = in[c][r] for c < cols (in: cols x rows), zero up to pp columns
+void add_transpose_pad(Phase& ph, int rows, int cols, int pp, const float* in, double* out);
+void add_gemm(Phase& ph, int r, int kk, int c, const double* X, const double* Y, double* Z);
+void add_rk4_combine(Phase& ph, int n, int m, double h, const double* A, const double* A2,
+                     const double* A3, const double* A4, const double* B, const double* AB,
+                     const double* A2B, const double* A3B, double* MT, double* Bbar);
+void add_abm_combine(Phase& ph, int n, int m, double h, const double* A, const double* A2,
+                     const double* B, const double* AB, double* MT, double* Bbar);
+void add_transpose(Phase& ph, int rows, int cols, const double* in, double* out);
+bool add_powers_small(Phase& ph, int na, int L, double* PT);
+void add_tables(Phase& ph, int na, int nap, int nrp, int nlev, int deltaBelow, const double* PT64,
+                float* PT32, float* PTm);
+void add_sample(Phase& ph, int m, const double* coef, double t0, double dt, long S, double sigma,
+                uint64_t seed, double* out);
+void add_startup(Phase& ph, int n, int m, double h, const float* A, const float* B,
+                 const float* x0, const double* Uh, double* X0);
+void add_start_outputs(Phase& ph, int n, int m, int p, int pp, bool useC, bool useD,
+                       const double* CT, const double* DT, const double* X0, const double* U,
+                       float* Y);
+// whether the leaves of this shape fit the matrix-core leaf kernel's LDS
+bool leaves_fit(int n, int m, int kq, int kg, long sL, long rs);
+bool add_leaves(Phase& ph, int n, int m, int kq, int kg, int na, int off, long nReal, long nPad,
+                const double* Bf, const LeafComb& G, const double* U, long sL, long oL, long rs,
+                float* X, bool writeZeros);
+bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nReal, long nPad,
+                     const double* BbT, const double* U, long sL, long oL, long rs, float* X,
+                     bool writeZeros);
+bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad, double h,
+                    const double* A64, const double* A264, float* X);
+bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nPad,
+                      const double* P64, float* X);
+// f64: the transposed FP64 power (ldp = na); else the padded FP32 table
+void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long d, long np,
+               const void* PT, float* S, float* X, float* Sn, long npn);
+void add_gather(Phase& ph, int na, long d, long np, const float* X, float* S);
+void add_top(Phase& ph, int na, int l0, int L, int l64, long nReal, const float* PT32,
+             const double* PT64, float* X, const double* seed, double* tot);
+void add_root(Phase& ph, int na, long last, const double* seed, float* X, double* tot);
+bool add_outputs(Phase& ph, int n, int m, int p, int pp, int na, int oX, bool useC, bool useD,
+                 long N, const double* CT, const double* DT, const float* X, const double* U,
+                 long sO, long oO, float* Y);
+// Launches L with its records at dArgs and its block table at dStart (device).
+void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dStart);
+}  // namespace g
 
 }  // namespace k
 }  // namespace parode

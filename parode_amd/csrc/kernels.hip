@@ -10,12 +10,20 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 #include <stdexcept>
 #include <string>
 
 namespace parode {
 namespace k {
+
+// The block a kernel body works on: (x, y) and the grid's width in x.  A
+// single-system launch passes its own blockIdx and gridDim; a grouped launch
+// (many systems in one grid) passes the block's place in its system's grid.
+struct VBlock {
+    long x, y, gx;
+};
 
 static void check_launch(const char* what)
 {
@@ -34,14 +42,20 @@ static inline unsigned blocks_for(long work, int block, long cap = 1L << 20)
 
 // ---------------------------------------------------------------- helpers
 template <typename T>
-__global__ void k_transpose(int rows, int cols, const T* __restrict__ in, T* __restrict__ out)
+__device__ __forceinline__ void d_transpose(const VBlock vb, int rows, int cols, const T* __restrict__ in, T* __restrict__ out)
 {
     const long total = (long)rows * cols;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
+    for (long idx = (long)vb.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)vb.gx * blockDim.x) {
         const long c = idx / rows, r = idx % rows;   // out[c][r] = in[r][c]
         out[idx] = in[r * cols + c];
     }
+}
+
+template <typename T>
+__global__ void k_transpose(int rows, int cols, const T* __restrict__ in, T* __restrict__ out)
+{
+    d_transpose<T>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, rows, cols, in, out);
 }
 
 template <typename T>
@@ -146,7 +160,7 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kGemmKsT = 32, kGemmKsKB = 32;
 
 template <int KSPLIT>
-__global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int c,
+__device__ __forceinline__ void d_gemm_f64_ks(const VBlock vb, int r, int kk, int c,
                                                               const double* __restrict__ X,
                                                               const double* __restrict__ Y,
                                                               double* __restrict__ Z)
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int grp = wave >> 2, wq = wave & 3;
     const int wr = wq >> 1, wc = wq & 1, l16 = lane & 15, q4 = lane >> 4;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
+    const int i0 = vb.y * T, j0 = vb.x * T;
     const int ar = tid / (KB / PA), ak = (tid % (KB / PA)) * PA;
     const int bk = tid / (T / PB), bj = (tid % (T / PB)) * PB;
     double ra[PA], rb[PB];
@@ -212,6 +226,15 @@ __global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int
     }
 }
 
+template <int KSPLIT>
+__global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int c,
+                                                              const double* __restrict__ X,
+                                                              const double* __restrict__ Y,
+                                                              double* __restrict__ Z)
+{
+    d_gemm_f64_ks<KSPLIT>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z);
+}
+
 void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
               bool mfma)
 {
@@ -244,7 +267,7 @@ void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double
 // four partial tiles are summed through LDS.  Operands come from L2 (the
 // matrices are at most a few MiB).
 
-__global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
+__device__ __forceinline__ void d_gemm_f64_mfma(const VBlock vb, int r, int kk, int c,
                                                        const double* __restrict__ X,
                                                        const double* __restrict__ Y,
                                                        double* __restrict__ Z)
@@ -252,7 +275,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
     __shared__ double part[3][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, q4 = lane >> 4;
-    const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+    const int i0 = vb.y * 16, j0 = vb.x * 16;
     const int ksteps = (kk + 3) / 4;
     const int per = (ksteps + 3) / 4;
     const int kb = wave * per, ke = (kb + per) < ksteps ? kb + per : ksteps;
@@ -287,11 +310,24 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
     }
 }
 
+__global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ Y,
+                                                       double* __restrict__ Z)
+{
+    d_gemm_f64_mfma(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z);
+}
+
+__device__ __forceinline__ void d_cast_f32_f64(const VBlock vb, long count, const float* __restrict__ in, double* __restrict__ out)
+{
+    for (long i = (long)vb.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)vb.gx * blockDim.x)
+        out[i] = (double)in[i];
+}
+
 __global__ void k_cast_f32_f64(long count, const float* __restrict__ in, double* __restrict__ out)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (long)gridDim.x * blockDim.x)
-        out[i] = (double)in[i];
+    d_cast_f32_f64(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, count, in, out);
 }
 
 void cast_f32_to_f64(hipStream_t s, long count, const float* in, double* out)
@@ -310,11 +346,16 @@ __global__ void k_fill(long count, T* __restrict__ out)
         out[i] = (T)0;
 }
 
+__device__ __forceinline__ void d_fill_f64(const VBlock vb, long count, double v, double* __restrict__ out)
+{
+    for (long i = (long)vb.x * blockDim.x + threadIdx.x; i < count;
+         i += (long)vb.gx * blockDim.x)
+        out[i] = v;
+}
+
 __global__ void k_fill_f64(long count, double v, double* __restrict__ out)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (long)gridDim.x * blockDim.x)
-        out[i] = v;
+    d_fill_f64(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, count, v, out);
 }
 
 void fill_zero_f32(hipStream_t s, long count, float* out)
@@ -333,10 +374,6 @@ void fill_f64(hipStream_t s, long count, double v, double* out)
 }
 
 // ------------------------------------------------------------- step maps
-struct Rk4Coef {
-    double h, c2, c3, c4, s1, s2, s3, t0, t1, t2, u0;
-};
-
 // Scalar factors evaluated left to right as in RungeKutta4thOrder.cpp:48-57.
 static Rk4Coef rk4_coef(double h)
 {
@@ -356,8 +393,10 @@ static Rk4Coef rk4_coef(double h)
     return c;
 }
 
+Rk4Coef rk4_coefficients(double h) { return rk4_coef(h); }
+
 template <typename TI>
-__global__ void k_rk4_combine(bool mirror, int n, int m, Rk4Coef k, const TI* __restrict__ A,
+__device__ __forceinline__ void d_rk4_combine(const VBlock vb, bool mirror, int n, int m, Rk4Coef k, const TI* __restrict__ A,
                               const TI* __restrict__ A2, const TI* __restrict__ A3,
                               const TI* __restrict__ A4, const TI* __restrict__ B,
                               const TI* __restrict__ AB, const TI* __restrict__ A2B,
@@ -365,7 +404,7 @@ __global__ void k_rk4_combine(bool mirror, int n, int m, Rk4Coef k, const TI* __
                               double* __restrict__ Bbar64, float* __restrict__ Bbar32)
 {
 #pragma clang fp contract(off)
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long idx = (long)vb.x * blockDim.x + threadIdx.x;
     long nn = (long)n * n;
     if (idx < nn) {
         int i = (int)(idx / n), j = (int)(idx % n);
@@ -393,6 +432,17 @@ __global__ void k_rk4_combine(bool mirror, int n, int m, Rk4Coef k, const TI* __
     }
 }
 
+template <typename TI>
+__global__ void k_rk4_combine(bool mirror, int n, int m, Rk4Coef k, const TI* __restrict__ A,
+                              const TI* __restrict__ A2, const TI* __restrict__ A3,
+                              const TI* __restrict__ A4, const TI* __restrict__ B,
+                              const TI* __restrict__ AB, const TI* __restrict__ A2B,
+                              const TI* __restrict__ A3B, double* __restrict__ MT64,
+                              double* __restrict__ Bbar64, float* __restrict__ Bbar32)
+{
+    d_rk4_combine<TI>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, mirror, n, m, k, A, A2, A3, A4, B, AB, A2B, A3B, MT64, Bbar64, Bbar32);
+}
+
 void rk4_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
                  const void* A3, const void* A4, const void* B, const void* AB, const void* A2B,
                  const void* A3B, double* M64, double* Bbar64, float* Bbar32)
@@ -411,10 +461,6 @@ void rk4_combine(hipStream_t s, bool mirror, int n, int m, double h, const void*
                            (const double*)A2B, (const double*)A3B, M64, Bbar64, Bbar32);
     check_launch("rk4_combine");
 }
-
-struct AbmCoef {
-    double a0A, a0A2, a1A, a1A2, a2A, a2A2, a3A2, bm1, b0B, b0AB, b1B, b1AB, b2B, b2AB, b3AB;
-};
 
 // AdamsMoulton.cpp:258-269, each factor evaluated left to right.
 static AbmCoef abm_coef(double h)
@@ -439,14 +485,16 @@ static AbmCoef abm_coef(double h)
     return c;
 }
 
+AbmCoef abm_coefficients(double h) { return abm_coef(h); }
+
 template <typename TI>
-__global__ void k_abm_combine(bool mirror, int n, int m, AbmCoef c, const TI* __restrict__ A,
+__device__ __forceinline__ void d_abm_combine(const VBlock vb, bool mirror, int n, int m, AbmCoef c, const TI* __restrict__ A,
                               const TI* __restrict__ A2, const TI* __restrict__ B,
                               const TI* __restrict__ AB, double* __restrict__ MT64,
                               double* __restrict__ Bbar64, float* __restrict__ Bbar32)
 {
 #pragma clang fp contract(off)
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long idx = (long)vb.x * blockDim.x + threadIdx.x;
     long nn = (long)n * n;
     const long na = 4L * n;
     auto rnd = [mirror](double v) { return mirror ? (double)(float)v : v; };
@@ -476,6 +524,15 @@ __global__ void k_abm_combine(bool mirror, int n, int m, AbmCoef c, const TI* __
         if (Bbar32) Bbar32[o] = f;
         Bbar64[o] = mirror ? (double)f : vals[k];
     }
+}
+
+template <typename TI>
+__global__ void k_abm_combine(bool mirror, int n, int m, AbmCoef c, const TI* __restrict__ A,
+                              const TI* __restrict__ A2, const TI* __restrict__ B,
+                              const TI* __restrict__ AB, double* __restrict__ MT64,
+                              double* __restrict__ Bbar64, float* __restrict__ Bbar32)
+{
+    d_abm_combine<TI>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, mirror, n, m, c, A, A2, B, AB, MT64, Bbar64, Bbar32);
 }
 
 void abm_combine(hipStream_t s, bool mirror, int n, int m, double h, const void* A, const void* A2,
@@ -556,14 +613,14 @@ void make_level_table_f64(hipStream_t s, int na, const double* P, bool delta, do
 // transposed FP64 powers (delta form below deltaBelow) as the unpadded FP32
 // table PT32 [na][na] (if not null) and the zero-padded MFMA table PTm
 // [nrp][nap] (if not null; else nap = nrp = na).
-__global__ void k_level_tables(int na, int nap, int nrp, int nlev, int deltaBelow,
+__device__ __forceinline__ void d_level_tables(const VBlock vb, int na, int nap, int nrp, int nlev, int deltaBelow,
                                const double* __restrict__ PT64, float* __restrict__ PT32,
                                float* __restrict__ PTm)
 {
     const long per = (long)nrp * nap, nn = (long)na * na;
     const long total = per * nlev;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
+    for (long idx = (long)vb.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)vb.gx * blockDim.x) {
         const int l = (int)(idx / per);
         const long r = idx - l * per;
         const int j = (int)(r / nap), i = (int)(r - (long)j * nap);
@@ -576,6 +633,13 @@ __global__ void k_level_tables(int na, int nap, int nrp, int nlev, int deltaBelo
         }
         if (PTm) PTm[idx] = v;
     }
+}
+
+__global__ void k_level_tables(int na, int nap, int nrp, int nlev, int deltaBelow,
+                               const double* __restrict__ PT64, float* __restrict__ PT32,
+                               float* __restrict__ PTm)
+{
+    d_level_tables(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, nap, nrp, nlev, deltaBelow, PT64, PT32, PTm);
 }
 
 void make_level_tables(hipStream_t s, int na, int nap, int nrp, int nlev, int deltaBelow,
@@ -595,7 +659,7 @@ void make_level_tables(hipStream_t s, int na, int nap, int nrp, int nlev, int de
 // to LDS and to PT[l + 1] after a barrier.  Replaces L launches of the GEMM.
 constexpr int kPowersSmallMax = 80;
 
-__global__ __launch_bounds__(1024) void k_powers_small(int na, int L, double* __restrict__ PT)
+__device__ __forceinline__ void d_powers_small(const VBlock vb, int na, int L, double* __restrict__ PT)
 {
     extern __shared__ double Ps[];
     const int n16 = (na + 15) / 16 * 16, pitch = n16 + 1;
@@ -641,6 +705,11 @@ __global__ __launch_bounds__(1024) void k_powers_small(int na, int L, double* __
     }
 }
 
+__global__ __launch_bounds__(1024) void k_powers_small(int na, int L, double* __restrict__ PT)
+{
+    d_powers_small(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, L, PT);
+}
+
 bool powers_small_supported(int na) { return na <= kPowersSmallMax; }
 
 void powers_small(hipStream_t s, int na, int L, double* PT)
@@ -680,7 +749,7 @@ struct Acc {
 constexpr int kLeafEB = 8;
 
 template <typename T, typename TU, typename TB>
-__global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
+__device__ __forceinline__ void d_leaves(const VBlock vb, int n, int m, int kq, int na, int off, long nReal,
                                                 long nPad, int tpeLog, int groups,
                                                 bool writeZeros, int zlo,
                                                 const TB* __restrict__ BbT,
@@ -697,7 +766,7 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
     // the last sample row a real leaf reads
     const long lastRow = (nReal - 1) * sL + oL + (long)(kq - 1) * rs;
     const long es = sL * m;                           // staged values between elements
-    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += (long)gridDim.x * perBlock) {
+    for (long eb = (long)vb.x * perBlock; eb < nPad; eb += (long)vb.gx * perBlock) {
         const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
         const long row0 = eb * sL + oL;
         long rows = (long)(cnt - 1) * sL + (long)(kq - 1) * rs + 1;
@@ -772,6 +841,17 @@ __global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, in
     }
 }
 
+template <typename T, typename TU, typename TB>
+__global__ __launch_bounds__(256) void k_leaves(int n, int m, int kq, int na, int off, long nReal,
+                                                long nPad, int tpeLog, int groups,
+                                                bool writeZeros, int zlo,
+                                                const TB* __restrict__ BbT,
+                                                const TU* __restrict__ U, long sL, long oL,
+                                                long rs, T* __restrict__ X)
+{
+    d_leaves<T, TU, TB>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m, kq, na, off, nReal, nPad, tpeLog, groups, writeZeros, zlo, BbT, U, sL, oL, rs, X);
+}
+
 // FP64 leaves on the matrix cores (v_mfma_f64_16x16x4f64), as the transposed
 // product V^T[e][r] = sum_c W[e][c] Bf[r][c] per tile of 16 leaves x 16 rows:
 //   A (16 leaves x 4 columns): lane l holds W[e0 + l%16][c0 + l/16]
@@ -792,7 +872,7 @@ constexpr int kLeafTiles = 4;
 constexpr int kLeafMaxQ = 5, kLeafMaxG = 2;
 
 template <typename T, typename TU, bool FAC>
-__global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, int na, int off,
+__device__ __forceinline__ void d_leaves_mfma(const VBlock vb, int n, int m, int kq, int na, int off,
                                                      long nReal, long nPad, int groups,
                                                      int pitch, unsigned magic, int kg,
                                                      LeafComb G, bool writeZeros, int zlo,
@@ -842,8 +922,8 @@ __global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, in
             if (t < vals) Us[r * pitch + (t - r * m)] = v[u];
         }
     };
-    const long stride = (long)gridDim.x * perBlock;
-    if (ahead && (long)blockIdx.x * perBlock < nPad) load_regs((long)blockIdx.x * perBlock);
+    const long stride = (long)vb.gx * perBlock;
+    if (ahead && (long)vb.x * perBlock < nPad) load_regs((long)vb.x * perBlock);
     // When every wave has one row tile for the whole launch and K fits eight
     // k-steps, its B fragments stay in registers across blocks.
     const bool keepB = rowTiles * groups <= 4 && ks <= 8;
@@ -857,7 +937,7 @@ __global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, in
             bk[u] = c < km ? BbT[(long)c * n + rr] : 0.0;
         }
     }
-    for (long eb = (long)blockIdx.x * perBlock; eb < nPad; eb += stride) {
+    for (long eb = (long)vb.x * perBlock; eb < nPad; eb += stride) {
         const int cnt = (nPad - eb) < perBlock ? (int)(nPad - eb) : (int)perBlock;
         const int vals = block_vals(eb);
         if (ahead) {
@@ -966,6 +1046,18 @@ __global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, in
         }
         __syncthreads();
     }
+}
+
+template <typename T, typename TU, bool FAC>
+__global__ __launch_bounds__(256, 3) void k_leaves_mfma(int n, int m, int kq, int na, int off,
+                                                     long nReal, long nPad, int groups,
+                                                     int pitch, unsigned magic, int kg,
+                                                     LeafComb G, bool writeZeros, int zlo,
+                                                     const double* __restrict__ BbT,
+                                                     const TU* __restrict__ U, long sL, long oL,
+                                                     long rs, T* __restrict__ X)
+{
+    d_leaves_mfma<T, TU, FAC>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m, kq, na, off, nReal, nPad, groups, pitch, magic, kg, G, writeZeros, zlo, BbT, U, sL, oL, rs, X);
 }
 
 // LDS bytes of one work-group of k_leaves_mfma: the staged sample rows, and
@@ -1712,12 +1804,8 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks(int n, int lc, long nReal
 // passes are those of k_abm_chunks, with A = [w1 w2] (16 chunks x 2 NP) and
 // B = [A A^2]^T rows of the wave's tile, formed from the FP64 products and
 // rounded once.
-struct AbmFactors {
-    float a[4], b[4];
-};
-
 template <int NP, bool PASS2>
-__global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long nReal,
+__device__ __forceinline__ void d_abm_chunks_fac(const VBlock vb, int n, int lc, long nReal,
                                                            const double* __restrict__ A64,
                                                            const double* __restrict__ A264,
                                                            AbmFactors f, float* __restrict__ X)
@@ -1732,7 +1820,7 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
     const int l16 = lane & 15, g = lane >> 4;
     const int r = wave * 16 + l16;
     const bool rowOk = r < n;
-    const long c0 = (long)blockIdx.x * CH;
+    const long c0 = (long)vb.x * CH;
     // B fragments: lane group g takes the padded states [g KS, (g+1) KS) of
     // [w1 w2]; padded state b NP + rr (b = 0: A, 1: A^2) is real when rr < n
     float bm[KS];
@@ -1846,6 +1934,15 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
         }
 }
 
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long nReal,
+                                                           const double* __restrict__ A64,
+                                                           const double* __restrict__ A264,
+                                                           AbmFactors f, float* __restrict__ X)
+{
+    d_abm_chunks_fac<NP, PASS2>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, lc, nReal, A64, A264, f, X);
+}
+
 // The factored ABM step for 128 < n <= 256, as two dependent products:
 //   x_new = x3 + A u + v_e,  u = w1 + A w2
 // so a CU holds only A (n x n FP32, in registers: a wave's 32 rows of A over
@@ -1858,7 +1955,7 @@ __global__ __launch_bounds__(NP * 4) void k_abm_chunks_fac(int n, int lc, long n
 // (LDS); barrier.  w2 and u have buffers of their own, so two barriers
 // per step suffice.  Lane layouts and passes are those of k_abm_chunks_fac.
 template <int NP, bool PASS2>
-__global__ __launch_bounds__(NP * 2) void k_abm_chunks_two(int n, int lc, long nReal,
+__device__ __forceinline__ void d_abm_chunks_two(const VBlock vb, int n, int lc, long nReal,
                                                            const double* __restrict__ A64,
                                                            AbmFactors f, float* __restrict__ X)
 {
@@ -1880,7 +1977,7 @@ __global__ __launch_bounds__(NP * 2) void k_abm_chunks_two(int n, int lc, long n
         rr[t] = wave * 32 + 16 * t + l16;
         ok[t] = rr[t] < n;
     }
-    const long c0 = (long)blockIdx.x * CH;
+    const long c0 = (long)vb.x * CH;
     float bm[2][KS];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
@@ -2008,7 +2105,23 @@ __global__ __launch_bounds__(NP * 2) void k_abm_chunks_two(int n, int lc, long n
             }
 }
 
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 2) void k_abm_chunks_two(int n, int lc, long nReal,
+                                                           const double* __restrict__ A64,
+                                                           AbmFactors f, float* __restrict__ X)
+{
+    d_abm_chunks_two<NP, PASS2>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, lc, nReal, A64, f, X);
+}
+
 bool abm_chunks_factored_supported(int n) { return n >= 1 && n <= 256; }
+
+AbmFactors abm_chunk_factors(double h)
+{
+    const AbmCoef c = abm_coef(h);
+    // blocks oldest first: x_{k-3} (A^2 only), x_{k-2}, x_{k-1}, x_k (+ I)
+    return AbmFactors{{0.0f, (float)c.a2A, (float)c.a1A, (float)c.a0A},
+                      {(float)c.a3A2, (float)c.a2A2, (float)c.a1A2, (float)c.a0A2}};
+}
 
 void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, long nPad,
                          double h, const double* A64, const double* A264, float* X)
@@ -2016,10 +2129,7 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
     if (!abm_chunks_factored_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 ||
         nReal > nPad)
         throw std::invalid_argument("abm_chunks_factored: bad shape");
-    const AbmCoef c = abm_coef(h);
-    // blocks oldest first: x_{k-3} (A^2 only), x_{k-2}, x_{k-1}, x_k (+ I)
-    const AbmFactors f = {{0.0f, (float)c.a2A, (float)c.a1A, (float)c.a0A},
-                          {(float)c.a3A2, (float)c.a2A2, (float)c.a1A2, (float)c.a0A2}};
+    const AbmFactors f = abm_chunk_factors(h);
     const long nChunks = nPad >> lc;
     const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
     auto launch = [&](auto np) {
@@ -2075,7 +2185,7 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
 //   element) and writes the state of every element of the chunk to its row,
 //   each after reading that element's leaf.
 template <int NP, bool PASS2>
-__global__ __launch_bounds__(NP * 4) void k_chunks_dense(int na, int lc, long S, long nReal,
+__device__ __forceinline__ void d_chunks_dense(const VBlock vb, int na, int lc, long S, long nReal,
                                                          const double* __restrict__ P64,
                                                          float* __restrict__ X)
 {
@@ -2090,7 +2200,7 @@ __global__ __launch_bounds__(NP * 4) void k_chunks_dense(int na, int lc, long S,
     const int l16 = lane & 15, g = lane >> 4;
     const int r = wave * 16 + l16;
     const bool rowOk = r < na;
-    const long c0 = (long)blockIdx.x * CH;
+    const long c0 = (long)vb.x * CH;
     auto pos = [&](long e) { return e * S + S - 1; };
     float bm[KS];
 #pragma unroll
@@ -2173,6 +2283,14 @@ __global__ __launch_bounds__(NP * 4) void k_chunks_dense(int na, int lc, long S,
         const int c = 4 * g + v;
         if (rowOk && c < cReal) X[pos((c0 + c + 1) * Lc - 1) * na + r] = xh[v] - xc[v];
     }
+}
+
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void k_chunks_dense(int na, int lc, long S, long nReal,
+                                                         const double* __restrict__ P64,
+                                                         float* __restrict__ X)
+{
+    d_chunks_dense<NP, PASS2>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, lc, S, nReal, P64, X);
 }
 
 bool chunks_dense_supported(int na) { return na >= 1 && na <= 256; }
@@ -2270,7 +2388,7 @@ struct KsTile<double, false> {
 };
 
 template <typename TT, typename TX, bool DOWN, bool DELTA, bool SMALL>
-__global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
+__device__ __forceinline__ void d_level_ks(const VBlock vb, int na, int ldp, long d, long np,
                                                   const TT* __restrict__ PT,
                                                   const TX* __restrict__ S, TX* __restrict__ X,
                                                   TX* __restrict__ Sn, long npn)
@@ -2282,8 +2400,8 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
     constexpr bool F32BIG = sizeof(TT) == 4 && !SMALL;
     __shared__ TT part[3][NACC][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long c0 = (long)blockIdx.x * T;
-    const int i0 = blockIdx.y * T;
+    const long c0 = (long)vb.x * T;
+    const int i0 = vb.y * T;
     // fragment coordinates: FP32 A(c = l%32, k = l/32), B(k = l/32, i = l%32);
     // FP64 A(c = l%16, k = l/16), B(k = l/16, i = l%16)
     const int lc = lane % T, lk = lane / T;
@@ -2369,16 +2487,31 @@ __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long 
     }
 }
 
+template <typename TT, typename TX, bool DOWN, bool DELTA, bool SMALL>
+__global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
+                                                  const TT* __restrict__ PT,
+                                                  const TX* __restrict__ S, TX* __restrict__ X,
+                                                  TX* __restrict__ Sn, long npn)
+{
+    d_level_ks<TT, TX, DOWN, DELTA, SMALL>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, ldp, d, np, PT, S, X, Sn, npn);
+}
+
 template <typename TX>
-__global__ void k_gather_rt(int na, long d, long np, const TX* __restrict__ X, TX* __restrict__ S)
+__device__ __forceinline__ void d_gather_rt(const VBlock vb, int na, long d, long np, const TX* __restrict__ X, TX* __restrict__ S)
 {
     const long total = np * na;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (long)gridDim.x * blockDim.x) {
+    for (long e = (long)vb.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)vb.gx * blockDim.x) {
         const long c = e / na;
         const int j = (int)(e - c * na);
         S[e] = X[((c + 1) * 2 * d - 1) * na + j];
     }
+}
+
+template <typename TX>
+__global__ void k_gather_rt(int na, long d, long np, const TX* __restrict__ X, TX* __restrict__ S)
+{
+    d_gather_rt<TX>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, d, np, X, S);
 }
 
 template <typename TT, typename TX>
@@ -2536,7 +2669,7 @@ __device__ __forceinline__ void top_update_mfma(bool down, int na, int SP, long 
 }
 
 template <typename TX, bool MFMA>
-__global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int l64, long nReal,
+__device__ __forceinline__ void d_top(const VBlock vb, int na, int l0, int L, int l64, long nReal,
                                                      const float* __restrict__ PT32,
                                                      const double* __restrict__ PT64,
                                                      TX* __restrict__ X,
@@ -2588,6 +2721,17 @@ __global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int 
     }
 }
 
+template <typename TX, bool MFMA>
+__global__ __launch_bounds__(kTopThreads) void k_top(int na, int l0, int L, int l64, long nReal,
+                                                     const float* __restrict__ PT32,
+                                                     const double* __restrict__ PT64,
+                                                     TX* __restrict__ X,
+                                                     const double* __restrict__ seed,
+                                                     double* __restrict__ tot, int CC)
+{
+    d_top<TX, MFMA>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, l0, L, l64, nReal, PT32, PT64, X, seed, tot, CC);
+}
+
 template <typename TX>
 void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float* PT32,
          const double* PT64, TX* X, const double* seed, double* tot)
@@ -2616,13 +2760,20 @@ template void top<double>(hipStream_t, int, int, int, int, long, const float*, c
 
 // -------------------------------------------------------------------- root
 template <typename TX, typename TS>
-__global__ void k_root(int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,
+__device__ __forceinline__ void d_root(const VBlock vb, int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,
                        double* __restrict__ tot)
 {
     for (int i = threadIdx.x; i < na; i += blockDim.x) {
         if (tot) tot[i] = (double)X[last * na + i];
         X[last * na + i] = (TX)seed[i];
     }
+}
+
+template <typename TX, typename TS>
+__global__ void k_root(int na, long last, const TS* __restrict__ seed, TX* __restrict__ X,
+                       double* __restrict__ tot)
+{
+    d_root<TX, TS>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, na, last, seed, X, tot);
 }
 
 template <typename TX, typename TS>
@@ -2771,15 +2922,14 @@ __global__ __launch_bounds__(256) void k_outputs_direct(int n, int m, int p, int
 constexpr int kOutLanes = 64;
 
 template <typename TX, typename TU>
-__global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
-    int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+__device__ __forceinline__ void d_outputs_lanes(const VBlock vb, int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
     const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
     const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
 {
     extern __shared__ unsigned char outl_smem[];
     TX* xs = reinterpret_cast<TX*>(outl_smem);   // [kOutLanes][n + 1]
     const int xp = n + 1;
-    const long stride = (long)gridDim.x * kOutLanes;
+    const long stride = (long)vb.gx * kOutLanes;
     // With n <= kOutLanes, thread c holds state c of every element of a block:
     // the next block's states are loaded into registers while this one computes.
     // (FP32 states only: FP64 ones would take twice the registers)
@@ -2795,8 +2945,8 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
             xv[u] = X[(e0 + q) * na + (long)oX * n + c];
         }
     };
-    if (ahead && (long)blockIdx.x * kOutLanes < N) load_x((long)blockIdx.x * kOutLanes);
-    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += stride) {
+    if (ahead && (long)vb.x * kOutLanes < N) load_x((long)vb.x * kOutLanes);
+    for (long e0 = (long)vb.x * kOutLanes; e0 < N; e0 += stride) {
         const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
         __syncthreads();
         if (ahead) {
@@ -2873,6 +3023,14 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
     }
 }
 
+template <typename TX, typename TU>
+__global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+    const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
+    const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
+{
+    d_outputs_lanes<TX, TU>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
+}
+
 // The same outputs for wide states: the states go through LDS in chunks of
 // kOutKc per element (64 x (kOutKc + 1) words per work-group, whatever n),
 // so many one-wave work-groups share a CU, and each lane keeps JG FP64 sums
@@ -2882,8 +3040,7 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_lanes(
 constexpr int kOutKc = 32;
 
 template <typename TX, typename TU, int JG>
-__global__ __launch_bounds__(kOutLanes) void k_outputs_kc(
-    int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+__device__ __forceinline__ void d_outputs_kc(const VBlock vb, int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
     const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
     const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
 {
@@ -2892,8 +3049,8 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_kc(
     const int t = threadIdx.x, cl = t & (kOutKc - 1), r0 = t / kOutKc;
     constexpr int RPL = kOutLanes / 2;    // staging loads per lane (two elements each)
     static_assert(kOutLanes == 2 * kOutKc, "two elements per staging load");
-    const long stride = (long)gridDim.x * kOutLanes;
-    for (long e0 = (long)blockIdx.x * kOutLanes; e0 < N; e0 += stride) {
+    const long stride = (long)vb.gx * kOutLanes;
+    for (long e0 = (long)vb.x * kOutLanes; e0 < N; e0 += stride) {
         const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
         const long e = e0 + (t < cnt ? t : cnt - 1);
         const TU* u = U + (e * sO + oO) * m;
@@ -2963,6 +3120,14 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_kc(
             }
         }
     }
+}
+
+template <typename TX, typename TU, int JG>
+__global__ __launch_bounds__(kOutLanes) void k_outputs_kc(int n, int m, int p, int pp, int na, int oX, bool useC, bool useD, long N,
+    const double* __restrict__ CT, const double* __restrict__ DT, const TX* __restrict__ X,
+    const TU* __restrict__ U, long sO, long oO, float* __restrict__ Y)
+{
+    d_outputs_kc<TX, TU, JG>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
 }
 
 __global__ void k_pad_cast(int rows, int cols, int pp, const float* __restrict__ in,
@@ -3083,7 +3248,7 @@ template void outputs<double, float>(hipStream_t, int, int, int, int, int, bool,
 // twelve derivative evaluations are chains of n dependent products per row,
 // which from global memory wait on a load each.
 template <typename TU>
-__global__ void k_abm_startup(bool mirror, int n, int m, double h, bool ldsA,
+__device__ __forceinline__ void d_abm_startup(const VBlock vb, bool mirror, int n, int m, double h, bool ldsA,
                               const float* __restrict__ A, const float* __restrict__ B,
                               const float* __restrict__ x0, const TU* __restrict__ Uh,
                               double* __restrict__ X0)
@@ -3164,6 +3329,15 @@ __global__ void k_abm_startup(bool mirror, int n, int m, double h, bool ldsA,
 }
 
 template <typename TU>
+__global__ void k_abm_startup(bool mirror, int n, int m, double h, bool ldsA,
+                              const float* __restrict__ A, const float* __restrict__ B,
+                              const float* __restrict__ x0, const TU* __restrict__ Uh,
+                              double* __restrict__ X0)
+{
+    d_abm_startup<TU>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, mirror, n, m, h, ldsA, A, B, x0, Uh, X0);
+}
+
+template <typename TU>
 void abm_startup(hipStream_t s, bool mirror, int n, int m, double h, const float* A,
                  const float* B, const float* x0, const TU* Uh, double* X0)
 {
@@ -3208,13 +3382,13 @@ __device__ inline uint64_t mix64(uint64_t& x)
     return z ^ (z >> 31);
 }
 
-__global__ void k_sample_family(int m, const double* __restrict__ coef, double t0, double dt,
+__device__ __forceinline__ void d_sample_family(const VBlock vb, int m, const double* __restrict__ coef, double t0, double dt,
                                 long s0, long S, double sigma, uint64_t seed,
                                 double* __restrict__ out)
 {
     const long total = S * m;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = (long)vb.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)vb.gx * blockDim.x) {
         const long sl = i / m;
         const int j = (int)(i - sl * m);
         const double* k = coef + 6 * j;
@@ -3233,6 +3407,13 @@ __global__ void k_sample_family(int m, const double* __restrict__ coef, double t
     }
 }
 
+__global__ void k_sample_family(int m, const double* __restrict__ coef, double t0, double dt,
+                                long s0, long S, double sigma, uint64_t seed,
+                                double* __restrict__ out)
+{
+    d_sample_family(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, m, coef, t0, dt, s0, S, sigma, seed, out);
+}
+
 void sample_family(hipStream_t s, int m, const double* coef, double t0, double dt, long s0,
                    long S, double sigma, uint64_t seed, double* out)
 {
@@ -3242,6 +3423,738 @@ void sample_family(hipStream_t s, int m, const double* coef, double t0, double d
                        coef, t0, dt, s0, S, sigma, seed, out);
     check_launch("sample_family");
 }
+
+
+// ====================================================== grouped launches
+// One grid for many systems.  System s of a launch owns the blocks
+// [start[s], start[s + 1]) and the argument record args[s]; a work-group
+// finds its system by a binary search over start (the same for every lane,
+// so it runs on scalar loads) and runs the same body as a single-system
+// launch, with its block's place in that system's grid.  The kernels a
+// system runs, and how its work is split into blocks, depend only on its own
+// shape, so its results do not depend on the other systems of the launch.
+namespace g {
+
+namespace {
+
+__device__ __forceinline__ int system_of(const unsigned* __restrict__ start, int nsys)
+{
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = nsys - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ VBlock block_of(const unsigned* __restrict__ start, int s, long gx)
+{
+    const long lb = (long)blockIdx.x - (long)start[s];
+    return VBlock{lb % gx, lb / gx, gx};
+}
+
+// Argument records (one per system of a launch; gx: the system's grid width).
+struct Prep {        // kind 0: out = (double)in; 1: out = v; 2: out[r][c] = in[c][r], c < cols, padded to pp
+    int kind, rows, cols, pp;
+    long count;
+    double v;
+    const float* in;
+    double* out;
+    long gx;
+};
+struct Gemm {
+    int r, kk, c;
+    const double *X, *Y;
+    double* Z;
+    long gx;
+};
+struct Rk4Comb {
+    int n, m;
+    Rk4Coef k;
+    const double *A, *A2, *A3, *A4, *B, *AB, *A2B, *A3B;
+    double *MT, *Bbar;
+    long gx;
+};
+struct AbmComb {
+    int n, m;
+    AbmCoef c;
+    const double *A, *A2, *B, *AB;
+    double *MT, *Bbar;
+    long gx;
+};
+struct Transpose {
+    int rows, cols;
+    const double* in;
+    double* out;
+    long gx;
+};
+struct Powers {
+    int na, L;
+    double* PT;
+    long gx;
+};
+struct Tables {
+    int na, nap, nrp, nlev, deltaBelow;
+    const double* PT64;
+    float *PT32, *PTm;
+    long gx;
+};
+struct Sample {
+    int m;
+    const double* coef;
+    double t0, dt;
+    long S;
+    double sigma;
+    uint64_t seed;
+    double* out;
+    long gx;
+};
+struct Startup {
+    int n, m;
+    double h;
+    bool ldsA;
+    const float *A, *B, *x0;
+    const double* Uh;
+    double* X0;
+    long gx;
+};
+struct StartOut {
+    int n, m, p, pp;
+    bool useC, useD;
+    const double *CT, *DT, *X0, *U;
+    float* Y;
+    long gx;
+};
+struct Leaves {
+    int n, m, kq, na, off;
+    long nReal, nPad;
+    int groups, pitch;
+    unsigned magic;
+    int kg;
+    LeafComb G;
+    bool writeZeros;
+    int zlo;
+    const double *BbT, *U;
+    long sL, oL, rs;
+    float* X;
+    long gx;
+};
+struct Chunks {      // ABM: n, A64, A264; dense: na (in n), P64 (in A64)
+    int n, lc;
+    long nReal;
+    const double *A64, *A264;
+    AbmFactors f;
+    float* X;
+    long gx;
+};
+struct Level {
+    int na, ldp;
+    long d, np;
+    const void* PT;
+    float *S, *X, *Sn;
+    long npn;
+    long gx;
+};
+struct Gather {
+    int na;
+    long d, np;
+    const float* X;
+    float* S;
+    long gx;
+};
+struct Top {
+    int na, l0, L, l64;
+    long nReal;
+    const float* PT32;
+    const double* PT64;
+    float* X;
+    const double* seed;
+    double* tot;
+    int CC;
+    long gx;
+};
+struct Root {
+    int na;
+    long last;
+    const double* seed;
+    float* X;
+    double* tot;
+    long gx;
+};
+struct Outputs {
+    int n, m, p, pp, na, oX;
+    bool useC, useD;
+    long N;
+    const double *CT, *DT;
+    const float* X;
+    const double* U;
+    long sO, oO;
+    float* Y;
+    long gx;
+};
+
+#define PARODE_GROUP(T)                          \
+    const int sys_ = system_of(st, ns);          \
+    const T& a = args[sys_];                     \
+    const VBlock vb = block_of(st, sys_, a.gx)
+
+__global__ __launch_bounds__(256) void kg_prep(const Prep* __restrict__ args,
+                                               const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Prep);
+    const long total = a.kind == 2 ? (long)a.rows * a.pp : a.count;
+    for (long i = vb.x * 256 + threadIdx.x; i < total; i += vb.gx * 256) {
+        double v;
+        if (a.kind == 0) {
+            v = (double)a.in[i];
+        } else if (a.kind == 1) {
+            v = a.v;
+        } else {
+            const long r = i / a.pp, c = i - r * a.pp;
+            v = c < a.cols ? (double)a.in[c * a.rows + r] : 0.0;
+        }
+        a.out[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void kg_gemm16(const Gemm* __restrict__ args,
+                                                 const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Gemm);
+    d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+}
+
+__global__ __launch_bounds__(512) void kg_gemm_ks2(const Gemm* __restrict__ args,
+                                                   const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Gemm);
+    d_gemm_f64_ks<2>(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+}
+
+__global__ __launch_bounds__(256) void kg_rk4_combine(const Rk4Comb* __restrict__ args,
+                                                      const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Rk4Comb);
+    d_rk4_combine<double>(vb, false, a.n, a.m, a.k, a.A, a.A2, a.A3, a.A4, a.B, a.AB, a.A2B,
+                          a.A3B, a.MT, a.Bbar, nullptr);
+}
+
+__global__ __launch_bounds__(256) void kg_abm_combine(const AbmComb* __restrict__ args,
+                                                      const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(AbmComb);
+    d_abm_combine<double>(vb, false, a.n, a.m, a.c, a.A, a.A2, a.B, a.AB, a.MT, a.Bbar, nullptr);
+}
+
+__global__ __launch_bounds__(256) void kg_transpose(const Transpose* __restrict__ args,
+                                                    const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Transpose);
+    d_transpose<double>(vb, a.rows, a.cols, a.in, a.out);
+}
+
+__global__ __launch_bounds__(1024) void kg_powers_small(const Powers* __restrict__ args,
+                                                        const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Powers);
+    d_powers_small(vb, a.na, a.L, a.PT);
+}
+
+__global__ __launch_bounds__(256) void kg_tables(const Tables* __restrict__ args,
+                                                 const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Tables);
+    d_level_tables(vb, a.na, a.nap, a.nrp, a.nlev, a.deltaBelow, a.PT64, a.PT32, a.PTm);
+}
+
+__global__ __launch_bounds__(256) void kg_sample(const Sample* __restrict__ args,
+                                                 const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Sample);
+    d_sample_family(vb, a.m, a.coef, a.t0, a.dt, 0, a.S, a.sigma, a.seed, a.out);
+}
+
+__global__ __launch_bounds__(256) void kg_startup(const Startup* __restrict__ args,
+                                                  const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Startup);
+    d_abm_startup<double>(vb, false, a.n, a.m, a.h, a.ldsA, a.A, a.B, a.x0, a.Uh, a.X0);
+}
+
+// The outputs of the ABM start-up states x_0, x_1, x_2 (the first n-blocks of
+// X0), whose inputs are rows 0, 2, 4 of the h/2-grid samples
+// (AdamsMoulton.cpp:106-116); FP64 sums, C terms then D terms.
+__global__ __launch_bounds__(256) void kg_start_outputs(const StartOut* __restrict__ args,
+                                                        const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(StartOut);
+    (void)vb;
+    for (int t = threadIdx.x; t < 3 * a.p; t += 256) {
+        const int i = t / a.p, j = t - i * a.p;
+        double v = 0.0;
+        if (a.useC)
+            for (int c = 0; c < a.n; ++c)
+                v = fma(a.CT[(long)c * a.pp + j], a.X0[(long)i * a.n + c], v);
+        if (a.useD)
+            for (int c = 0; c < a.m; ++c)
+                v = fma(a.DT[(long)c * a.pp + j], a.U[(long)(2 * i) * a.m + c], v);
+        a.Y[(long)i * a.p + j] = (float)v;
+    }
+}
+
+template <bool FAC>
+__global__ __launch_bounds__(256, 3) void kg_leaves(const Leaves* __restrict__ args,
+                                                    const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Leaves);
+    d_leaves_mfma<float, double, FAC>(vb, a.n, a.m, a.kq, a.na, a.off, a.nReal, a.nPad, a.groups,
+                                      a.pitch, a.magic, a.kg, a.G, a.writeZeros, a.zlo, a.BbT,
+                                      a.U, a.sL, a.oL, a.rs, a.X);
+}
+
+__global__ __launch_bounds__(256) void kg_leaves_simt(const Leaves* __restrict__ args,
+                                                      const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Leaves);
+    // (Leaves::pitch holds the thread groups' width, log 2)
+    d_leaves<float, double, double>(vb, a.n, a.m, a.kq, a.na, a.off, a.nReal, a.nPad, a.pitch,
+                                    a.groups, a.writeZeros, a.zlo, a.BbT, a.U, a.sL, a.oL, a.rs,
+                                    a.X);
+}
+
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void kg_chunks_fac(const Chunks* __restrict__ args,
+                                                        const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Chunks);
+    d_abm_chunks_fac<NP, PASS2>(vb, a.n, a.lc, a.nReal, a.A64, a.A264, a.f, a.X);
+}
+
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 2) void kg_chunks_two(const Chunks* __restrict__ args,
+                                                        const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Chunks);
+    d_abm_chunks_two<NP, PASS2>(vb, a.n, a.lc, a.nReal, a.A64, a.f, a.X);
+}
+
+template <int NP, bool PASS2>
+__global__ __launch_bounds__(NP * 4) void kg_chunks_dense(const Chunks* __restrict__ args,
+                                                          const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Chunks);
+    d_chunks_dense<NP, PASS2>(vb, a.n, a.lc, 1, a.nReal, a.A64, a.X);
+}
+
+// FP32 levels on 16 x 16 tiles of the padded MFMA table (TT = float), FP64
+// levels on 16 x 16 tiles of the transposed FP64 power (TT = double).
+template <typename TT, bool DOWN, bool DELTA>
+__global__ __launch_bounds__(256) void kg_level(const Level* __restrict__ args,
+                                                const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Level);
+    d_level_ks<TT, float, DOWN, DELTA, sizeof(TT) == 4>(
+        vb, a.na, a.ldp, a.d, a.np, static_cast<const TT*>(a.PT), a.S, a.X, DOWN ? a.Sn : nullptr,
+        DOWN ? a.npn : 0L);
+}
+
+__global__ __launch_bounds__(256) void kg_gather(const Gather* __restrict__ args,
+                                                 const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Gather);
+    d_gather_rt<float>(vb, a.na, a.d, a.np, a.X, a.S);
+}
+
+template <bool MFMA>
+__global__ __launch_bounds__(kTopThreads) void kg_top(const Top* __restrict__ args,
+                                                      const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Top);
+    d_top<float, MFMA>(vb, a.na, a.l0, a.L, a.l64, a.nReal, a.PT32, a.PT64, a.X, a.seed, a.tot,
+                       a.CC);
+}
+
+__global__ __launch_bounds__(256) void kg_root(const Root* __restrict__ args,
+                                               const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Root);
+    d_root<float, double>(vb, a.na, a.last, a.seed, a.X, a.tot);
+}
+
+__global__ __launch_bounds__(kOutLanes) void kg_out_lanes(const Outputs* __restrict__ args,
+                                                          const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Outputs);
+    d_outputs_lanes<float, double>(vb, a.n, a.m, a.p, a.pp, a.na, a.oX, a.useC, a.useD, a.N, a.CT,
+                                   a.DT, a.X, a.U, a.sO, a.oO, a.Y);
+}
+
+template <int JG>
+__global__ __launch_bounds__(kOutLanes) void kg_out_kc(const Outputs* __restrict__ args,
+                                                       const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Outputs);
+    d_outputs_kc<float, double, JG>(vb, a.n, a.m, a.p, a.pp, a.na, a.oX, a.useC, a.useD, a.N,
+                                    a.CT, a.DT, a.X, a.U, a.sO, a.oO, a.Y);
+}
+
+#undef PARODE_GROUP
+
+// The record of the launch (kind, flag) in phase ph, created on first use.
+template <typename A>
+Launch& slot(Phase& ph, int kind, bool flag = false)
+{
+    Launch& L = ph[{kind, flag ? 1 : 0}];
+    if (L.kind < 0) {
+        L.kind = kind;
+        L.flag = flag;
+        L.argSize = sizeof(A);
+    }
+    if (L.argSize != sizeof(A)) throw std::logic_error("grouped launch: argument type mismatch");
+    return L;
+}
+
+template <typename A>
+void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds = 0)
+{
+    if (blocks < 1) return;
+    Launch& L = slot<A>(ph, kind, flag);
+    const size_t o = L.args.size();
+    L.args.resize(o + sizeof(A));
+    std::memcpy(L.args.data() + o, &a, sizeof(A));
+    const unsigned long next = (unsigned long)L.start.back() + (unsigned long)blocks;
+    if (next > 0x7fffffffUL) throw std::length_error("grouped launch: too many blocks");
+    L.start.push_back((unsigned)next);
+    L.lds = std::max(L.lds, lds);
+}
+
+}  // namespace
+
+void add_cast(Phase& ph, long count, const float* in, double* out)
+{
+    if (count <= 0) return;
+    const long b = blocks_for(count, 256, 4096);
+    append(ph, kPrep, false, Prep{0, 0, 0, 0, count, 0.0, in, out, b}, b);
+}
+
+void add_fill(Phase& ph, long count, double v, double* out)
+{
+    if (count <= 0) return;
+    const long b = blocks_for(count, 256, 4096);
+    append(ph, kPrep, false, Prep{1, 0, 0, 0, count, v, nullptr, out, b}, b);
+}
+
+void add_transpose_pad(Phase& ph, int rows, int cols, int pp, const float* in, double* out)
+{
+    if (rows <= 0 || pp <= 0) return;
+    const long b = blocks_for((long)rows * pp, 256, 1024);
+    append(ph, kPrep, false, Prep{2, rows, cols, pp, 0, 0.0, in, out, b}, b);
+}
+
+void add_gemm(Phase& ph, int r, int kk, int c, const double* X, const double* Y, double* Z)
+{
+    if (r <= 0 || c <= 0) return;
+    if (r >= 320 && c >= 320 && kk >= 64) {
+        const long gx = (c + kGemmKsT - 1) / kGemmKsT, gy = (r + kGemmKsT - 1) / kGemmKsT;
+        const size_t stage = (size_t)(kGemmKsT * (kGemmKsKB + 1) + kGemmKsKB * (kGemmKsT + 1));
+        append(ph, kGemmKs2, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy,
+               std::max<size_t>(stage, 1 * 4 * 4 * 64) * sizeof(double));
+    } else {
+        const long gx = (c + 15) / 16, gy = (r + 15) / 16;
+        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy);
+    }
+}
+
+void add_rk4_combine(Phase& ph, int n, int m, double h, const double* A, const double* A2,
+                     const double* A3, const double* A4, const double* B, const double* AB,
+                     const double* A2B, const double* A3B, double* MT, double* Bbar)
+{
+    const long b = blocks_for((long)n * n + (long)n * m, 256);
+    append(ph, kRk4Combine, false,
+           Rk4Comb{n, m, rk4_coef(h), A, A2, A3, A4, B, AB, A2B, A3B, MT, Bbar, b}, b);
+}
+
+void add_abm_combine(Phase& ph, int n, int m, double h, const double* A, const double* A2,
+                     const double* B, const double* AB, double* MT, double* Bbar)
+{
+    const long b = blocks_for((long)n * n + (long)n * m, 256);
+    append(ph, kAbmCombine, false, AbmComb{n, m, abm_coef(h), A, A2, B, AB, MT, Bbar, b}, b);
+}
+
+void add_transpose(Phase& ph, int rows, int cols, const double* in, double* out)
+{
+    if ((long)rows * cols <= 0) return;
+    const long b = blocks_for((long)rows * cols, 256, 4096);
+    append(ph, kTranspose, false, Transpose{rows, cols, in, out, b}, b);
+}
+
+bool add_powers_small(Phase& ph, int na, int L, double* PT)
+{
+    if (na > kPowersSmallMax) return false;
+    if (L <= 0) return true;
+    const int n16 = (na + 15) / 16 * 16;
+    append(ph, kPowersSmall, false, Powers{na, L, PT, 1}, 1,
+           (size_t)n16 * (n16 + 1) * sizeof(double));
+    return true;
+}
+
+void add_tables(Phase& ph, int na, int nap, int nrp, int nlev, int deltaBelow, const double* PT64,
+                float* PT32, float* PTm)
+{
+    if (nlev <= 0) return;
+    if (!PTm) nap = nrp = na;
+    const long b = blocks_for((long)nrp * nap * nlev, 256, 16384);
+    append(ph, kTables, false, Tables{na, nap, nrp, nlev, deltaBelow, PT64, PT32, PTm, b}, b);
+}
+
+void add_sample(Phase& ph, int m, const double* coef, double t0, double dt, long S, double sigma,
+                uint64_t seed, double* out)
+{
+    if (S * m <= 0) return;
+    const long b = blocks_for(S * m, 256, 8192);
+    append(ph, kSample, false, Sample{m, coef, t0, dt, S, sigma, seed, out, b}, b);
+}
+
+void add_startup(Phase& ph, int n, int m, double h, const float* A, const float* B,
+                 const float* x0, const double* Uh, double* X0)
+{
+    size_t lds = (size_t)6 * n * sizeof(double);
+    const size_t ldsA = (size_t)n * (n + 1) * sizeof(float);
+    const bool useA = lds + ldsA <= 65536;
+    if (useA) lds += ldsA;
+    append(ph, kStartup, false, Startup{n, m, h, useA, A, B, x0, Uh, X0, 1}, 1, lds);
+}
+
+void add_start_outputs(Phase& ph, int n, int m, int p, int pp, bool useC, bool useD,
+                       const double* CT, const double* DT, const double* X0, const double* U,
+                       float* Y)
+{
+    if (p <= 0) return;
+    append(ph, kStartOut, false, StartOut{n, m, p, pp, useC, useD, CT, DT, X0, U, Y, 1}, 1);
+}
+
+bool leaves_fit(int n, int m, int kq, int kg, long sL, long rs)
+{
+    if (m <= 0 || m >= 65536) return false;
+    const int rowTiles = (n + 15) / 16;
+    int groups = rowTiles >= 4 ? 1 : 4 / rowTiles;
+    const int pitch = m | 1;
+    while (groups > 1 &&
+           leaves_mfma_lds(16L * kLeafTiles * groups, m, kq, kg, sL, rs, pitch) > 65536)
+        groups >>= 1;
+    const long perBlock = 16L * kLeafTiles * groups;
+    const size_t lds = leaves_mfma_lds(perBlock, m, kq, kg, sL, rs, pitch);
+    return lds <= 65536 && lds / sizeof(double) < 65536 && perBlock * m < 65536;
+}
+
+bool add_leaves(Phase& ph, int n, int m, int kq, int kg, int na, int off, long nReal, long nPad,
+                const double* Bf, const LeafComb& G, const double* U, long sL, long oL, long rs,
+                float* X, bool writeZeros)
+{
+    if (m <= 0 || nReal < 1) return false;
+    const int rowTiles = (n + 15) / 16;
+    int groups = rowTiles >= 4 ? 1 : 4 / rowTiles;
+    const int pitch = m | 1;
+    while (groups > 1 &&
+           leaves_mfma_lds(16L * kLeafTiles * groups, m, kq, kg, sL, rs, pitch) > 65536)
+        groups >>= 1;
+    const long perBlock = 16L * kLeafTiles * groups;
+    const size_t lds = leaves_mfma_lds(perBlock, m, kq, kg, sL, rs, pitch);
+    if (lds > 65536 || m >= 65536 || lds / sizeof(double) >= 65536 || perBlock * m >= 65536)
+        return false;
+    const unsigned magic = (unsigned)(((1ULL << 32) + m - 1) / m);
+    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 2048), 1);
+    append(ph, kg > 0 ? kLeavesFac : kLeavesPlain, false,
+           Leaves{n, m, kq, na, off, nReal, nPad, groups, pitch, magic, kg, G, writeZeros,
+                  off / 32 * 32, Bf, U, sL, oL, rs, X, b},
+           b, lds);
+    return true;
+}
+
+// The vector-ALU leaves (k_leaves) from B-bar itself, for input counts whose
+// staged samples do not fit the matrix-core kernel: thread groups of 2^tpeLog
+// (in Leaves::pitch) lanes, Leaves::groups of them per work-group.
+bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nReal, long nPad,
+                     const double* BbT, const double* U, long sL, long oL, long rs, float* X,
+                     bool writeZeros)
+{
+    if (m <= 0 || nReal < 1) return false;
+    int tpeLog = 0;
+    while ((1 << tpeLog) < n && tpeLog < 8) ++tpeLog;
+    int groups = 256 >> tpeLog;
+    auto lds_for = [&](int g) {
+        return (size_t)(((long)g * kLeafEB - 1) * sL + (long)(kq - 1) * rs + 1) * m *
+               sizeof(double);
+    };
+    while (groups > 1 && lds_for(groups) > 65536) groups >>= 1;
+    if (lds_for(groups) > 65536) return false;
+    const long perBlock = (long)groups * kLeafEB;
+    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16), 1);
+    append(ph, kLeavesSimt, false,
+           Leaves{n, m, kq, na, off, nReal, nPad, groups, tpeLog, 0u, 0, LeafComb{}, writeZeros,
+                  off / 32 * 32, BbT, U, sL, oL, rs, X, b},
+           b, lds_for(groups));
+    return true;
+}
+
+bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad, double h,
+                    const double* A64, const double* A264, float* X)
+{
+    if (!abm_chunks_factored_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 ||
+        nReal > nPad)
+        return false;
+    const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
+    const Chunks a{n, lc, nReal, A64, A264, abm_chunk_factors(h), X, b};
+    const int np = n <= 16 ? 16 : n <= 32 ? 32 : n <= 64 ? 64 : n <= 128 ? 128 : n <= 160 ? 160
+                 : n <= 192 ? 192 : n <= 224 ? 224 : 256;
+    const int kind = np == 16 ? kChunksFac16 : np == 32 ? kChunksFac32 : np == 64 ? kChunksFac64
+                   : np == 128 ? kChunksFac128 : np == 160 ? kChunksTwo160
+                   : np == 192 ? kChunksTwo192 : np == 224 ? kChunksTwo224 : kChunksTwo256;
+    const size_t lds = np <= 128 ? (size_t)2 * kChunkCols * (2 * np + 4) * sizeof(float)
+                                 : (size_t)2 * kChunkCols * (np + 4) * sizeof(float);
+    append(ph, kind, pass2, a, b, lds);
+    return true;
+}
+
+bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nPad,
+                      const double* P64, float* X)
+{
+    if (!chunks_dense_supported(na) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
+        return false;
+    const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
+    const int np = na <= 16 ? 16 : na <= 32 ? 32 : na <= 64 ? 64 : na <= 128 ? 128 : 256;
+    const int kind = np == 16 ? kDense16 : np == 32 ? kDense32 : np == 64 ? kDense64
+                   : np == 128 ? kDense128 : kDense256;
+    append(ph, kind, pass2, Chunks{na, lc, nReal, P64, nullptr, AbmFactors{}, X, b}, b,
+           (size_t)kChunkCols * (np + 4) * sizeof(float));
+    return true;
+}
+
+void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long d, long np,
+               const void* PT, float* S, float* X, float* Sn, long npn)
+{
+    if (np <= 0) return;
+    if (!f64 && ldp % 32 != 0) throw std::invalid_argument("add_level: FP32 table must be padded");
+    if (down && (!S || (Sn && (Sn == S || npn < 0 || npn > 2 * np))))
+        throw std::invalid_argument("add_level: bad copy buffers");
+    const long gx = (np + 15) / 16, gy = (na + 15) / 16;
+    const int kind = f64 ? kLevel64 : delta ? kLevel32Delta : kLevel32;
+    append(ph, kind, down, Level{na, ldp, d, np, PT, S, X, Sn, Sn ? npn : 0, gx}, gx * gy);
+}
+
+void add_gather(Phase& ph, int na, long d, long np, const float* X, float* S)
+{
+    if (np <= 0) return;
+    const long b = blocks_for(np * na, 256, 4096);
+    append(ph, kGather, false, Gather{na, d, np, X, S, b}, b);
+}
+
+void add_top(Phase& ph, int na, int l0, int L, int l64, long nReal, const float* PT32,
+             const double* PT64, float* X, const double* seed, double* tot)
+{
+    const int rowThreads = na < kTopThreads ? na : kTopThreads;
+    const int groups = kTopThreads / rowThreads;
+    const int SP = na >= 16 ? na + 1 : na;
+    int CC = na >= 16 ? std::min(kTopMfmaCols, 6144 / SP)
+                      : std::min(groups * kTopMaxCols, 6144 / na);
+    if (CC < 1) CC = 1;
+    append(ph, na >= 16 ? kTopMfma : kTopSimt, false,
+           Top{na, l0, L, l64, nReal, PT32, PT64, X, seed, tot, CC, 1}, 1,
+           (size_t)CC * SP * sizeof(double));
+}
+
+void add_root(Phase& ph, int na, long last, const double* seed, float* X, double* tot)
+{
+    append(ph, kRoot, false, Root{na, last, seed, X, tot, 1}, 1);
+}
+
+bool add_outputs(Phase& ph, int n, int m, int p, int pp, int na, int oX, bool useC, bool useD,
+                 long N, const double* CT, const double* DT, const float* X, const double* U,
+                 long sO, long oO, float* Y)
+{
+    if (N <= 0) return true;
+    if (p <= 0 || pp % 8 != 0 || pp < p) return false;
+    const long b = std::max<long>(std::min<long>((N + kOutLanes - 1) / kOutLanes, 4096), 1);
+    const Outputs a{n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y, b};
+    if (n <= kOutLanes) {
+        append(ph, kOutputsLanes, false, a, b, (size_t)kOutLanes * (n + 1) * sizeof(float));
+    } else {
+        const int kind = pp % 32 == 0 ? kOutputsKc32 : pp % 24 == 0 ? kOutputsKc24
+                       : pp % 16 == 0 ? kOutputsKc16 : kOutputsKc8;
+        append(ph, kind, false, a, b);
+    }
+    return true;
+}
+
+void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dStart)
+{
+    const int ns = L.nsys();
+    if (ns <= 0 || L.blocks() == 0) return;
+    const dim3 grid(L.blocks());
+    const size_t lds = L.lds;
+    const bool f = L.flag;
+#define PARODE_RUN(KERNEL, T, THREADS)                                                        \
+    hipLaunchKernelGGL((KERNEL), grid, dim3(THREADS), lds, s, static_cast<const T*>(dArgs),   \
+                       dStart, ns)
+#define PARODE_RUN2(K0, K1, T, THREADS)      \
+    do {                                     \
+        if (f)                               \
+            PARODE_RUN(K1, T, THREADS);      \
+        else                                 \
+            PARODE_RUN(K0, T, THREADS);      \
+    } while (0)
+    switch (L.kind) {
+    case kPrep: PARODE_RUN(kg_prep, Prep, 256); break;
+    case kGemm16: PARODE_RUN(kg_gemm16, Gemm, 256); break;
+    case kGemmKs2: PARODE_RUN(kg_gemm_ks2, Gemm, 512); break;
+    case kRk4Combine: PARODE_RUN(kg_rk4_combine, Rk4Comb, 256); break;
+    case kAbmCombine: PARODE_RUN(kg_abm_combine, AbmComb, 256); break;
+    case kTranspose: PARODE_RUN(kg_transpose, Transpose, 256); break;
+    case kPowersSmall: PARODE_RUN(kg_powers_small, Powers, 1024); break;
+    case kTables: PARODE_RUN(kg_tables, Tables, 256); break;
+    case kSample: PARODE_RUN(kg_sample, Sample, 256); break;
+    case kStartup: PARODE_RUN(kg_startup, Startup, 256); break;
+    case kStartOut: PARODE_RUN(kg_start_outputs, StartOut, 256); break;
+    case kLeavesPlain: PARODE_RUN(kg_leaves<false>, Leaves, 256); break;
+    case kLeavesFac: PARODE_RUN(kg_leaves<true>, Leaves, 256); break;
+    case kLeavesSimt: PARODE_RUN(kg_leaves_simt, Leaves, 256); break;
+    case kChunksFac16: PARODE_RUN2((kg_chunks_fac<16, false>), (kg_chunks_fac<16, true>), Chunks, 64); break;
+    case kChunksFac32: PARODE_RUN2((kg_chunks_fac<32, false>), (kg_chunks_fac<32, true>), Chunks, 128); break;
+    case kChunksFac64: PARODE_RUN2((kg_chunks_fac<64, false>), (kg_chunks_fac<64, true>), Chunks, 256); break;
+    case kChunksFac128: PARODE_RUN2((kg_chunks_fac<128, false>), (kg_chunks_fac<128, true>), Chunks, 512); break;
+    case kChunksTwo160: PARODE_RUN2((kg_chunks_two<160, false>), (kg_chunks_two<160, true>), Chunks, 320); break;
+    case kChunksTwo192: PARODE_RUN2((kg_chunks_two<192, false>), (kg_chunks_two<192, true>), Chunks, 384); break;
+    case kChunksTwo224: PARODE_RUN2((kg_chunks_two<224, false>), (kg_chunks_two<224, true>), Chunks, 448); break;
+    case kChunksTwo256: PARODE_RUN2((kg_chunks_two<256, false>), (kg_chunks_two<256, true>), Chunks, 512); break;
+    case kDense16: PARODE_RUN2((kg_chunks_dense<16, false>), (kg_chunks_dense<16, true>), Chunks, 64); break;
+    case kDense32: PARODE_RUN2((kg_chunks_dense<32, false>), (kg_chunks_dense<32, true>), Chunks, 128); break;
+    case kDense64: PARODE_RUN2((kg_chunks_dense<64, false>), (kg_chunks_dense<64, true>), Chunks, 256); break;
+    case kDense128: PARODE_RUN2((kg_chunks_dense<128, false>), (kg_chunks_dense<128, true>), Chunks, 512); break;
+    case kDense256: PARODE_RUN2((kg_chunks_dense<256, false>), (kg_chunks_dense<256, true>), Chunks, 1024); break;
+    case kLevel32: PARODE_RUN2((kg_level<float, false, false>), (kg_level<float, true, false>), Level, 256); break;
+    case kLevel32Delta: PARODE_RUN2((kg_level<float, false, true>), (kg_level<float, true, true>), Level, 256); break;
+    case kLevel64: PARODE_RUN2((kg_level<double, false, false>), (kg_level<double, true, false>), Level, 256); break;
+    case kGather: PARODE_RUN(kg_gather, Gather, 256); break;
+    case kTopMfma: PARODE_RUN(kg_top<true>, Top, kTopThreads); break;
+    case kTopSimt: PARODE_RUN(kg_top<false>, Top, kTopThreads); break;
+    case kRoot: PARODE_RUN(kg_root, Root, 256); break;
+    case kOutputsLanes: PARODE_RUN(kg_out_lanes, Outputs, kOutLanes); break;
+    case kOutputsKc8: PARODE_RUN(kg_out_kc<8>, Outputs, kOutLanes); break;
+    case kOutputsKc16: PARODE_RUN(kg_out_kc<16>, Outputs, kOutLanes); break;
+    case kOutputsKc24: PARODE_RUN(kg_out_kc<24>, Outputs, kOutLanes); break;
+    case kOutputsKc32: PARODE_RUN(kg_out_kc<32>, Outputs, kOutLanes); break;
+    default: throw std::logic_error("grouped launch: unknown kernel");
+    }
+#undef PARODE_RUN2
+#undef PARODE_RUN
+    check_launch("grouped launch");
+}
+
+}  // namespace g
 
 }  // namespace k
 }  // namespace parode

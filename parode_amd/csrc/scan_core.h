@@ -22,6 +22,8 @@
 
 namespace parode {
 
+class BatchSolve;
+
 enum Method { kRK4 = 0, kABM = 1 };
 constexpr int kKsMinStates = 16;   // smallest scan size for the K-split levels
 enum Precision { kPlain32 = 0, kMixed = 1, kFp64 = 2 };
@@ -127,6 +129,7 @@ public:
     static void clear_schedule();
 
 private:
+    friend class BatchSolve;   // plans grouped launches over the buffers below
     bool level_is_f64(int l) const;
     bool level_is_delta(int l) const;
     void level(hipStream_t s, bool down, int l, long np);

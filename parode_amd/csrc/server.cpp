@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "batch.h"
 #include "config.h"
 #include "kernels.h"
 
@@ -14,12 +15,12 @@ namespace {
 
 size_t up(size_t b) { return Arena::round(b); }
 
-// Byte offsets of one request's arrays in its device block (io) and pinned
-// staging block.  io: [A B C D x0 | coef | U | Y]; staging holds the same
-// prefix (and U when the samples come from the host), then Y.
+// Byte offsets of one request's uploaded arrays (relative to its place in the
+// group's upload region, the same in pinned staging and on the device):
+// [A B C D x0 | coef | U when the samples come from the host].
 struct Layout {
-    size_t A, B, C, D, x0, coef, U, Y, ioBytes;
-    size_t stY, stBytes, h2d;
+    size_t A, B, C, D, x0, coef, U, up;
+    size_t uBytes, yBytes;
     int nOut;
 };
 
@@ -35,15 +36,12 @@ Layout layout(const ServeRequest& r)
     L.D = o;  o += r.zD ? 0 : p * m * 4;
     L.x0 = o; o += n * 4;
     o = up(o);
-    L.coef = o; o += m * 6 * 8;
+    L.coef = o; o += r.samples ? 0 : m * 6 * 8;
     o = up(o);
-    L.U = o;  o += (size_t)(2 * r.points + 1) * m * 8;
-    o = up(o);
-    L.Y = o;  o += (size_t)r.points * L.nOut * 4;
-    L.ioBytes = up(o);
-    L.h2d = r.samples ? L.Y : L.U;
-    L.stY = L.h2d;
-    L.stBytes = up(L.stY + (size_t)r.points * L.nOut * 4);
+    L.uBytes = up((size_t)(2 * r.points + 1) * m * 8);
+    L.U = o;  o += r.samples ? L.uBytes : 0;
+    L.up = up(o);
+    L.yBytes = up((size_t)r.points * L.nOut * 4);
     return L;
 }
 
@@ -56,27 +54,63 @@ void check_range(const char* what, size_t off, size_t bytes, size_t blockBytes)
 
 }  // namespace
 
+// A pinned staging block shared by the requests of one group: it returns to
+// the pinned arena when the last of them has handed over its outputs.
+struct Server::Stage {
+    Arena* arena = nullptr;
+    char* p = nullptr;
+    size_t bytes = 0;
+    ~Stage()
+    {
+        if (arena) arena->give(p);
+    }
+};
+
 struct Server::Job {
     long id = 0;
     ServeRequest r;
-    std::vector<float> mats;
+    std::vector<float> mats;     // A B C D x0 at their Layout offsets
     std::vector<double> coef;
     Layout L{};
-    std::unique_ptr<ScanCore> core;
-    char* io = nullptr;
-    char* stage = nullptr;
-    int lane = -1;
-    hipEvent_t done = nullptr;
+    std::shared_ptr<Stage> stage;
+    size_t stY = 0;              // its outputs in the stage (host outputs)
     Clock::time_point tSubmit, tAdmit, tDone;
-    int status = 1;   // 1 = pending
+    int status = 1;              // 1 = pending
+};
+
+// The unit of GPU work: requests admitted together onto one lane.
+struct Server::Group {
+    std::vector<std::shared_ptr<Job>> jobs;
+    bool grouped = false;
+    int lane = -1;
+    std::unique_ptr<BatchSolve> batch;   // grouped requests
+    std::unique_ptr<ScanCore> core;      // one request of any shape
+    char* io = nullptr;                  // device: uploads, samples, host-output staging
+    char* desc = nullptr;                // device: the batch's launch records
+    hipEvent_t done = nullptr;
+    void release(Arena* arena)
+    {
+        if (done) (void)hipEventDestroy(done);
+        done = nullptr;
+        batch.reset();
+        core.reset();
+        arena->give(io);
+        arena->give(desc);
+        io = desc = nullptr;
+    }
 };
 
 size_t Server::footprint(const ServeRequest& r, int precision)
 {
+    const Layout L = layout(r);
+    size_t io = L.up + (r.samples ? 0 : L.uBytes) + (r.yDevice ? 0 : L.yBytes);
+    if (precision == kMixed &&
+        BatchSolve::eligible(r.solver, r.n, r.m, r.p, r.zC, r.zD, r.points))
+        return BatchSolve::footprint(r.solver, r.n, r.m, r.p, r.points) + io;
     ScanOptions opt;
     opt.precision = precision;
     const long nScan = r.solver == kRK4 ? r.points : r.points - 3;
-    return ScanCore::estimate_footprint(r.solver, r.n, r.m, nScan, opt, r.p) + layout(r).ioBytes;
+    return ScanCore::estimate_footprint(r.solver, r.n, r.m, nScan, opt, r.p) + io;
 }
 
 Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
@@ -90,8 +124,7 @@ Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
     load_.assign(lanes, 0);
     for (auto& s : lanes_) PARODE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     st_.arenaBytes = arena_->capacity();
-    workers_ = lanes < 4 ? lanes : 4;
-    for (int w = 0; w < workers_; ++w) threads_.emplace_back([this, w] { loop(w); });
+    worker_ = std::thread([this] { loop(); });
 }
 
 Server::~Server()
@@ -101,8 +134,7 @@ Server::~Server()
         stop_ = true;
     }
     work_cv_.notify_all();
-    for (auto& t : threads_)
-        if (t.joinable()) t.join();
+    if (worker_.joinable()) worker_.join();
     for (auto& s : lanes_) (void)hipStreamDestroy(s);
 }
 
@@ -139,7 +171,8 @@ long Server::submit(const ServeRequest& r0)
         ++st_.failed;
         return j->id;
     }
-    if (footprint(r, precision_) > arena_->capacity() || j->L.stBytes > pinned_->capacity()) {
+    const size_t stage = j->L.up + (r.yDevice ? 0 : j->L.yBytes);
+    if (footprint(r, precision_) > arena_->capacity() || stage > pinned_->capacity()) {
         j->status = ParODE_NotEnoughResources;
         j->tAdmit = j->tDone = j->tSubmit;
         ++st_.failed;
@@ -150,153 +183,253 @@ long Server::submit(const ServeRequest& r0)
     return j->id;
 }
 
-bool Server::admit(Job& j, int lane)
+bool Server::grouped(const Job& j) const
 {
     const ServeRequest& r = j.r;
-    const Layout& L = j.L;
-    ScanOptions opt;
-    opt.precision = precision_;
-    const long nScan = r.solver == kRK4 ? r.points : r.points - 3;
+    return precision_ == kMixed &&
+           BatchSolve::eligible(r.solver, r.n, r.m, r.p, r.zC, r.zD, r.points);
+}
+
+bool Server::start(Group& g)
+{
+    hipStream_t s = lanes_[g.lane];
+    // device io block: [uploads][samples made on the device][host outputs];
+    // pinned stage: [uploads][host outputs][launch records]
+    size_t upTot = 0, uTot = 0, yTot = 0;
+    std::vector<size_t> upOff(g.jobs.size()), uOff(g.jobs.size()), yOff(g.jobs.size());
+    for (size_t i = 0; i < g.jobs.size(); ++i) {
+        const Job& j = *g.jobs[i];
+        upOff[i] = upTot;
+        upTot += j.L.up;
+        uOff[i] = uTot;
+        uTot += j.r.samples || j.r.m == 0 ? 0 : j.L.uBytes;
+        yOff[i] = yTot;
+        yTot += j.r.yDevice ? 0 : j.L.yBytes;
+    }
+    const size_t ioBytes = upTot + uTot + yTot;
+    g.io = static_cast<char*>(arena_->take(ioBytes));
+    if (!g.io) return false;
+    char* dU = g.io + upTot;
+    char* dYs = dU + uTot;
+    auto fdev = [&](size_t i, size_t off) {
+        return reinterpret_cast<const float*>(g.io + upOff[i] + off);
+    };
+    auto u_of = [&](size_t i) {
+        const Job& j = *g.jobs[i];
+        return reinterpret_cast<double*>(j.r.samples ? g.io + upOff[i] + j.L.U : dU + uOff[i]);
+    };
+    auto y_of = [&](size_t i) {
+        const Job& j = *g.jobs[i];
+        return j.r.yDevice ? j.r.Y : reinterpret_cast<float*>(dYs + yOff[i]);
+    };
+    size_t descBytes = 0;
     try {
-        j.core.reset(new ScanCore(r.solver, r.n, r.m, r.p, r.zC, r.zD, nScan, opt, arena_.get()));
+        if (g.grouped) {
+            std::vector<BatchSystem> sys(g.jobs.size());
+            for (size_t i = 0; i < g.jobs.size(); ++i) {
+                const Job& j = *g.jobs[i];
+                BatchSystem& q = sys[i];
+                q.method = j.r.solver;
+                q.n = j.r.n;
+                q.m = j.r.m;
+                q.p = j.r.p;
+                q.points = j.r.points;
+                q.t0 = j.r.t0;
+                q.h = j.r.h;
+                q.A = fdev(i, j.L.A);
+                q.B = fdev(i, j.L.B);
+                q.C = j.r.zC ? nullptr : fdev(i, j.L.C);
+                q.D = j.r.zD ? nullptr : fdev(i, j.L.D);
+                q.x0 = fdev(i, j.L.x0);
+                q.coef = j.r.samples ? nullptr
+                                     : reinterpret_cast<const double*>(g.io + upOff[i] + j.L.coef);
+                q.sigma = j.r.sigma;
+                q.seed = j.r.seed;
+                q.U = u_of(i);
+                q.Y = y_of(i);
+            }
+            g.batch.reset(new BatchSolve(sys, arena_.get()));
+            descBytes = g.batch->descriptor_bytes();
+            if (descBytes) {
+                g.desc = static_cast<char*>(arena_->take(descBytes));
+                if (!g.desc) throw std::bad_alloc();
+            }
+        } else {
+            const Job& j = *g.jobs[0];
+            ScanOptions opt;
+            opt.precision = precision_;
+            const long nScan = j.r.solver == kRK4 ? j.r.points : j.r.points - 3;
+            g.core.reset(new ScanCore(j.r.solver, j.r.n, j.r.m, j.r.p, j.r.zC, j.r.zD, nScan, opt,
+                                      arena_.get()));
+        }
     } catch (std::bad_alloc&) {
-        j.core.reset();
+        g.release(arena_.get());
         return false;
     }
-    j.io = static_cast<char*>(arena_->take(L.ioBytes));
-    j.stage = j.io ? static_cast<char*>(pinned_->take(L.stBytes)) : nullptr;
-    if (!j.io || !j.stage) {
-        arena_->give(j.io);
-        j.io = nullptr;
-        j.core.reset();
+    auto stage = std::make_shared<Stage>();
+    stage->bytes = upTot + yTot + Arena::round(descBytes);
+    stage->p = static_cast<char*>(pinned_->take(stage->bytes));
+    if (!stage->p) {
+        g.release(arena_.get());
         return false;
     }
-    hipStream_t s = lanes_[lane];
+    stage->arena = pinned_.get();
     // every host copy and every transfer stays inside its own block
-    const size_t uBytes = (size_t)(2 * r.points + 1) * r.m * 8;
-    const size_t yBytes = (size_t)r.points * L.nOut * 4;
-    check_range("staged matrices", 0, j.mats.size() * 4, L.stBytes);
-    check_range("staged coefficients", L.coef, j.coef.size() * 8, L.stBytes);
-    if (r.samples && r.m) check_range("staged samples", L.U, uBytes, L.stBytes);
-    check_range("upload", 0, L.h2d, std::min(L.stBytes, L.ioBytes));
-    check_range("device outputs", L.Y, yBytes, L.ioBytes);
-    check_range("staged outputs", L.stY, yBytes, L.stBytes);
-    std::memcpy(j.stage, j.mats.data(), j.mats.size() * 4);
-    if (!j.coef.empty()) std::memcpy(j.stage + L.coef, j.coef.data(), j.coef.size() * 8);
-    if (r.samples && r.m) std::memcpy(j.stage + L.U, r.samples, uBytes);
-    PARODE_HIP(hipMemcpyAsync(j.io, j.stage, L.h2d, hipMemcpyHostToDevice, s));
-    if (!r.samples && r.m)
-        k::sample_family(s, r.m, reinterpret_cast<const double*>(j.io + L.coef), r.t0, 0.5 * r.h,
-                         0, 2 * r.points + 1, r.sigma, r.seed,
-                         reinterpret_cast<double*>(j.io + L.U));
-    auto f = [&](size_t off) { return reinterpret_cast<const float*>(j.io + off); };
-    float* dY = reinterpret_cast<float*>(j.io + L.Y);
-    j.core->solve(s, r.points, r.h, f(L.A), r.m ? f(L.B) : nullptr, r.zC ? nullptr : f(L.C),
-                  r.zD ? nullptr : f(L.D), f(L.x0), j.io + L.U, true, dY);
-    PARODE_HIP(hipMemcpyAsync(j.stage + L.stY, dY, yBytes, hipMemcpyDeviceToHost, s));
-    PARODE_HIP(hipEventCreateWithFlags(&j.done, hipEventDisableTiming));
-    PARODE_HIP(hipEventRecord(j.done, s));
-    j.lane = lane;
-    j.tAdmit = Clock::now();
+    for (size_t i = 0; i < g.jobs.size(); ++i) {
+        Job& j = *g.jobs[i];
+        const ServeRequest& r = j.r;
+        check_range("staged matrices", upOff[i], j.mats.size() * 4, upTot);
+        check_range("staged coefficients", upOff[i] + j.L.coef, j.coef.size() * 8, upTot);
+        std::memcpy(stage->p + upOff[i], j.mats.data(), j.mats.size() * 4);
+        if (!j.coef.empty())
+            std::memcpy(stage->p + upOff[i] + j.L.coef, j.coef.data(), j.coef.size() * 8);
+        if (r.samples && r.m) {
+            const size_t ub = (size_t)(2 * r.points + 1) * r.m * 8;
+            check_range("staged samples", upOff[i] + j.L.U, ub, upTot);
+            std::memcpy(stage->p + upOff[i] + j.L.U, r.samples, ub);
+        }
+        if (!r.yDevice) {
+            check_range("device outputs", upTot + uTot + yOff[i], j.L.yBytes, ioBytes);
+            check_range("staged outputs", upTot + yOff[i], j.L.yBytes, upTot + yTot);
+        }
+        j.stage = stage;
+        j.stY = upTot + yOff[i];
+    }
+    check_range("launch records", upTot + yTot, descBytes, stage->bytes);
+    PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
+    if (g.grouped) {
+        g.batch->run(s, stage->p + upTot + yTot, g.desc);
+    } else {
+        const Job& j = *g.jobs[0];
+        const ServeRequest& r = j.r;
+        double* U = u_of(0);
+        if (!r.samples && r.m)
+            k::sample_family(s, r.m, reinterpret_cast<const double*>(g.io + upOff[0] + j.L.coef), r.t0,
+                             0.5 * r.h, 0, 2 * r.points + 1, r.sigma, r.seed, U);
+        g.core->solve(s, r.points, r.h, fdev(0, j.L.A), r.m ? fdev(0, j.L.B) : nullptr,
+                      r.zC ? nullptr : fdev(0, j.L.C), r.zD ? nullptr : fdev(0, j.L.D),
+                      fdev(0, j.L.x0), U, true, y_of(0));
+    }
+    if (yTot) PARODE_HIP(hipMemcpyAsync(stage->p + upTot, dYs, yTot, hipMemcpyDeviceToHost, s));
+    PARODE_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    PARODE_HIP(hipEventRecord(g.done, s));
+    const auto now = Clock::now();
+    for (auto& j : g.jobs) j->tAdmit = now;
     return true;
 }
 
-void Server::finish(Job& j, int status)
+void Server::finish(Group& g, int status)
 {
-    if (j.done) (void)hipEventDestroy(j.done);
-    j.done = nullptr;
-    j.core.reset();
-    arena_->give(j.io);
-    j.io = nullptr;
-    if (status != ParODE_OK) {   // nothing to copy out
-        pinned_->give(j.stage);
-        j.stage = nullptr;
+    g.release(arena_.get());
+    std::lock_guard<std::mutex> lk(mu_);
+    if (g.lane >= 0) {
+        --load_[g.lane];
+        running_ -= (long)g.jobs.size();
     }
-    std::lock_guard<std::mutex> g(mu_);
-    if (j.lane >= 0) {
-        --load_[j.lane];
-        --running_;
+    const auto now = Clock::now();
+    for (auto& jp : g.jobs) {
+        Job& j = *jp;
+        if (status != ParODE_OK || j.r.yDevice) j.stage.reset();   // nothing to copy out
+        if (j.stage) ++staged_;
+        j.tDone = now;
+        j.status = status;
+        if (status == ParODE_OK)
+            ++st_.completed;
+        else
+            ++st_.failed;
     }
-    if (j.stage) ++staged_;
-    j.tDone = Clock::now();
-    j.status = status;
-    if (status == ParODE_OK)
-        ++st_.completed;
-    else
-        ++st_.failed;
     done_cv_.notify_all();
 }
 
-void Server::loop(int worker)
+void Server::loop()
 {
     if (hipSetDevice(device_) != hipSuccess) return;
-    std::vector<std::shared_ptr<Job>> running;   // this worker's requests on the GPU
+    std::vector<std::unique_ptr<Group>> running;
     for (;;) {
         bool progressed = false;
-        // retire the finished requests
+        // retire the finished groups
         for (size_t i = 0; i < running.size();) {
-            Job& j = *running[i];
-            const hipError_t e = hipEventQuery(j.done);
+            const hipError_t e = hipEventQuery(running[i]->done);
             if (e == hipErrorNotReady) {
                 ++i;
                 continue;
             }
-            finish(j, e == hipSuccess ? ParODE_OK : ParODE_NotEnoughResources);
+            finish(*running[i], e == hipSuccess ? ParODE_OK : ParODE_NotEnoughResources);
             running.erase(running.begin() + i);
             progressed = true;
         }
-        // admit in arrival order while one of this worker's lanes has room
+        // admit the head of the queue while a lane has room
         for (;;) {
-            std::shared_ptr<Job> j;
-            int lane = -1;
+            std::unique_ptr<Group> g(new Group);
             {
-                std::lock_guard<std::mutex> g(mu_);
+                std::lock_guard<std::mutex> lk(mu_);
                 if (queue_.empty()) break;
-                for (int l = worker; l < (int)lanes_.size(); l += workers_)
+                int lane = -1;
+                for (int l = 0; l < (int)lanes_.size(); ++l)
                     if (load_[l] < depth_ && (lane < 0 || load_[l] < load_[lane])) lane = l;
                 if (lane < 0) break;
-                j = queue_.front();
-                queue_.pop_front();
+                g->lane = lane;
+                g->grouped = grouped(*queue_.front());
+                // a group: the eligible requests at the head, in order, while
+                // the free device memory holds them
+                size_t bytes = 0;
+                const size_t room = arena_->largest_free();
+                while (!queue_.empty() && g->jobs.size() < (g->grouped ? groupCap_ : 1)) {
+                    const Job& j = *queue_.front();
+                    if (grouped(j) != g->grouped) break;
+                    const size_t b = footprint(j.r, precision_) + 64 * Arena::kAlign;
+                    if (!g->jobs.empty() && bytes + b > room) break;
+                    bytes += b;
+                    g->jobs.push_back(queue_.front());
+                    queue_.pop_front();
+                }
                 ++load_[lane];
-                ++active_;
+                running_ += (long)g->jobs.size();
             }
             bool ok = false;
             int err = ParODE_OK;
             try {
-                ok = admit(*j, lane);
+                ok = start(*g);
             } catch (std::exception& ex) {
                 last_error() = ex.what();
                 err = ParODE_NotEnoughResources;
-                (void)hipStreamSynchronize(lanes_[lane]);   // nothing of it may still run
+                (void)hipStreamSynchronize(lanes_[g->lane]);   // nothing of it may still run
+                g->release(arena_.get());
             }
-            if (!ok) {
-                bool requeued = false;
-                {
-                    std::lock_guard<std::mutex> g(mu_);
-                    --load_[lane];
-                    --active_;
-                    // no memory now: back to the head of the queue while
-                    // anything else holds memory (running, being admitted, or
-                    // outputs not yet collected); else it can never fit
-                    if (err == ParODE_OK &&
-                        (running_ > 0 || active_ > 0 || staged_ > 0 || !running.empty())) {
-                        queue_.push_front(j);
-                        requeued = true;
-                    }
-                }
-                if (requeued) break;
-                finish(*j, err == ParODE_OK ? ParODE_NotEnoughResources : err);
+            if (ok) {
+                std::lock_guard<std::mutex> lk(mu_);
+                st_.peakRunning = std::max(st_.peakRunning, running_);
+                st_.arenaPeak = std::max(st_.arenaPeak, arena_->in_use());
+                ++st_.groups;
+                if (g->grouped) st_.groupedRequests += (long)g->jobs.size();
+                st_.largestGroup = std::max(st_.largestGroup, (long)g->jobs.size());
+                if (g->grouped && groupCap_ < 1024) groupCap_ *= 2;
+                running.push_back(std::move(g));
                 progressed = true;
                 continue;
             }
+            bool requeued = false;
             {
-                std::lock_guard<std::mutex> g(mu_);
-                --active_;
-                ++running_;
-                st_.peakRunning = std::max(st_.peakRunning, running_);
-                st_.arenaPeak = std::max(st_.arenaPeak, arena_->in_use());
+                std::lock_guard<std::mutex> lk(mu_);
+                --load_[g->lane];
+                running_ -= (long)g->jobs.size();
+                // no memory now: back to the head of the queue (a smaller group
+                // next time) while anything else holds memory; else a single
+                // request can never fit
+                if (err == ParODE_OK &&
+                    (g->jobs.size() > 1 || !running.empty() || staged_ > 0)) {
+                    for (auto it = g->jobs.rbegin(); it != g->jobs.rend(); ++it)
+                        queue_.push_front(*it);
+                    if (g->jobs.size() > 1) groupCap_ = std::max<size_t>(1, g->jobs.size() / 2);
+                    requeued = true;
+                }
             }
-            running.push_back(j);
+            if (requeued) {
+                if (g->jobs.size() > 1) continue;   // retry at once with the smaller cap
+                break;
+            }
+            g->lane = -1;
+            finish(*g, err == ParODE_OK ? ParODE_NotEnoughResources : err);
             progressed = true;
         }
         std::unique_lock<std::mutex> lk(mu_);
@@ -327,14 +460,13 @@ int Server::wait(long ticket, double* latencyMs, double* queueMs)
     // the outputs leave pinned staging on the caller's thread
     if (j->stage) {
         if (j->status == ParODE_OK)
-            std::memcpy(j->r.Y, j->stage + j->L.stY, (size_t)j->r.points * j->L.nOut * 4);
-        pinned_->give(j->stage);
-        j->stage = nullptr;
+            std::memcpy(j->r.Y, j->stage->p + j->stY, (size_t)j->r.points * j->L.nOut * 4);
+        j->stage.reset();
         {
             std::lock_guard<std::mutex> g(mu_);
             --staged_;
         }
-        work_cv_.notify_all();   // staging space came back
+        work_cv_.notify_all();   // staging space may have come back
     }
     return j->status;
 }
@@ -407,6 +539,40 @@ int ParODE_ServerSubmit(void* server, int solver, int nStates, int nInputs, int 
     }
 }
 
+int ParODE_ServerSubmitDevice(void* server, int solver, int nStates, int nInputs, int nOutputs,
+                              int zeroC, int zeroD, long points, double tStart, double tStep,
+                              const float* A, const float* B, const float* C, const float* D,
+                              const float* x0, const double* coef, double sigma,
+                              unsigned long long seed, const double* samples, float* dY,
+                              long* ticket)
+{
+    if (!server || !ticket) return ParODE_InvalidSystem;
+    parode::ServeRequest r;
+    r.solver = solver;
+    r.n = nStates;
+    r.m = nInputs;
+    r.p = nOutputs;
+    r.zC = zeroC != 0;
+    r.zD = zeroD != 0;
+    r.points = points;
+    r.t0 = tStart;
+    r.h = tStep;
+    r.A = A; r.B = B; r.C = C; r.D = D; r.x0 = x0;
+    r.coef = coef;
+    r.sigma = sigma;
+    r.seed = seed;
+    r.samples = samples;
+    r.Y = dY;
+    r.yDevice = true;
+    try {
+        *ticket = static_cast<parode::Server*>(server)->submit(r);
+        return ParODE_OK;
+    } catch (std::exception& e) {
+        parode::last_error() = e.what();
+        return ParODE_NotEnoughResources;
+    }
+}
+
 int ParODE_ServerWait(void* server, long ticket, double* timesMs)
 {
     if (!server) return ParODE_InvalidSystem;
@@ -433,6 +599,16 @@ int ParODE_ServerStats(void* server, long* stats)
     const long v[8] = {s.submitted, s.completed, s.failed, s.running, s.queued, s.peakRunning,
                        (long)s.arenaBytes, (long)s.arenaPeak};
     for (int i = 0; i < 8; ++i) stats[i] = v[i];
+    return ParODE_OK;
+}
+
+int ParODE_ServerGroupStats(void* server, long* stats)
+{
+    if (!server || !stats) return ParODE_InvalidSystem;
+    parode::ServeStats s = static_cast<parode::Server*>(server)->stats();
+    stats[0] = s.groups;
+    stats[1] = s.groupedRequests;
+    stats[2] = s.largestGroup;
     return ParODE_OK;
 }
 

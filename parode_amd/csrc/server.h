@@ -1,18 +1,20 @@
 // Continuous-batching request loop for LTI solves on one GPU.
 //
 // Requests (one system each: method, shape, horizon, matrices, inputs) are
-// submitted from any thread and run on worker threads that own the GPU work.
-// A worker admits queued requests in arrival order whenever a lane
-// (a HIP stream) has room and the device arena can hold the request's
-// footprint (scan trajectory, FP64 power table, level tables, inputs and
-// outputs).  An admitted request's whole solve is enqueued on its lane without
-// host synchronisation, so requests on different lanes overlap on the device
-// and new arrivals are admitted while earlier ones run.  Several worker
-// threads share the admission queue, each driving its own lanes, so kernel
-// launches are issued in parallel.  When a request's GPU work ends, its device
-// memory returns to the arena at once; its outputs wait in pinned staging
-// until the caller's wait() copies them out.  Inputs are either host samples
-// on the h/2 grid or the coefficients of the family u_j = a sin(w t + phi) +
+// submitted from any thread.  One worker thread owns the GPU work.  Whenever
+// a lane (a HIP stream) has room, it admits the requests waiting at the head
+// of the queue, in arrival order, as one group: with mixed precision, every
+// request the grouped kernels cover (BatchSolve::eligible) joins the group
+// (up to the free device memory), and the group's scans run as one set of
+// grouped launches; any other request runs alone through its own ScanCore.
+// A group's work is enqueued on its lane without host synchronisation, so
+// groups on different lanes overlap on the device and new arrivals are
+// admitted while earlier groups run.  A group's inputs go up in one copy
+// (pinned staging -> device), its launch records in another; when its GPU
+// work ends, its device memory returns to the arena at once; host outputs
+// wait in pinned staging until the caller's wait() copies them out, device
+// outputs are written in place.  Inputs are either host samples on the h/2
+// grid or the coefficients of the family u_j = a sin(w t + phi) +
 // b cos(w t + phi) + c + d t (+ noise), sampled in FP64 on the device.
 #pragma once
 #include <chrono>
@@ -39,13 +41,15 @@ struct ServeRequest {
     double sigma = 0.0;
     uint64_t seed = 0;
     const double* samples = nullptr;  // host, (2*points+1) x m
-    float* Y = nullptr;               // host, points x nOut
+    float* Y = nullptr;               // points x nOut: host memory, or device memory when yDevice
+    bool yDevice = false;
 };
 
 struct ServeStats {
     long submitted = 0, completed = 0, failed = 0, running = 0, queued = 0;
     long peakRunning = 0;
     size_t arenaBytes = 0, arenaPeak = 0;
+    long groups = 0, groupedRequests = 0, largestGroup = 0;
 };
 
 class Server {
@@ -67,12 +71,17 @@ public:
 
 private:
     struct Job;
+    struct Group;
+    struct Stage;
     using Clock = std::chrono::steady_clock;
-    void loop(int worker);
-    bool admit(Job& j, int lane);
-    void finish(Job& j, int status);
+    void loop();
+    bool grouped(const Job& j) const;
+    // enqueues the group's work on lanes_[lane]; false: no memory for it now
+    bool start(Group& g);
+    void finish(Group& g, int status);
 
-    int device_ = 0, precision_ = 0, depth_ = 2, workers_ = 1;
+    int device_ = 0, precision_ = 0, depth_ = 2;
+    size_t groupCap_ = 1024;   // requests per group (halved while memory is short)
     std::vector<hipStream_t> lanes_;
     std::vector<int> load_;
     std::unique_ptr<Arena> arena_, pinned_;
@@ -80,10 +89,10 @@ private:
     std::condition_variable work_cv_, done_cv_;
     std::deque<std::shared_ptr<Job>> queue_;
     std::unordered_map<long, std::shared_ptr<Job>> jobs_;
-    long next_ = 1, running_ = 0, active_ = 0, staged_ = 0;
+    long next_ = 1, running_ = 0, staged_ = 0;
     bool stop_ = false;
     ServeStats st_;
-    std::vector<std::thread> threads_;
+    std::thread worker_;
 };
 
 }  // namespace parode
