@@ -254,7 +254,16 @@ def group_leg(args, torch, api, systems, dev):
     st = [g.execute_timed(sp) for _ in range(max(3, args.steps))]
     stages = {k: float(np.median([x[k] for x in st])) for k in st[0]}
     total = float(np.median([sum(x.values()) for x in st]))
-    out = {"launches": g.launches, "bytes": g.footprint, "stage_ms": stages, "ms": total}
+    # the same solve with the power tables and the start-up on the side stream
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps)
+    ev0.record(stream)
+    for _ in range(reps):
+        g.execute(sp)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    out = {"launches": g.launches, "bytes": g.footprint, "stage_ms": stages,
+           "ms_stages_serial": total, "ms": ev0.elapsed_time(ev1) / reps}
     g.close()
     del keep
     torch.cuda.empty_cache()
@@ -362,7 +371,7 @@ def single_legs(args, torch, api, synth, dev):
             plan.execute(*ptrs, sp)
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 20
+        reps = args.single_reps
         ev0.record(stream)
         for _ in range(reps):
             plan.execute(*ptrs, sp)
@@ -370,7 +379,7 @@ def single_legs(args, torch, api, synth, dev):
         torch.cuda.synchronize()
         ms = ev0.elapsed_time(ev1) / reps
         phases, lat = [], []
-        for _ in range(20):
+        for _ in range(args.single_reps):
             t0 = time.perf_counter()
             phases.append(plan.execute_timed(*ptrs, sp))
             lat.append((time.perf_counter() - t0) * 1e3)
@@ -455,7 +464,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-single", action="store_true")
-    ap.add_argument("--only", choices=["group", "served", "plain"], default=None,
+    ap.add_argument("--single-reps", type=int, default=20)
+    ap.add_argument("--only", choices=["group", "served", "plain", "single"], default=None,
                     help="run one leg alone (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -475,6 +485,9 @@ def main():
     if args.only == "group":
         g = group_leg(args, torch, api, systems, dev)
         print(json.dumps({"group": g}))
+        return
+    if args.only == "single":
+        print(json.dumps({"single": single_legs(args, torch, api, synth, dev)}))
         return
     if args.only == "plain":
         ms, _ = plain_leg(args, torch, api, systems, dev)

@@ -52,6 +52,12 @@
  *                        or_simulate_f64 within their rounding at short
  *                        horizons, and equals the FP64 recurrence driven by the
  *                        mirror's float M, Bbar and samples
+ *   or_simulate_*_nsb    batches of NSB elements: with sampled inputs (no
+ *                        restart) they equal one batch to rounding, and the
+ *                        padded outputs of the power-of-two last batch are
+ *                        dropped; with UFunc inputs every batch's samples
+ *                        equal the float grid restarted at float(t_b) (checked
+ *                        element by element against a direct evaluation)
  *   or_gauss_noise       moments of the generated samples
  *   all modes            golden fixtures (the .npz files of tests/golden) guard against
  *                        regressions once the checks above pass

@@ -18,9 +18,10 @@
  * "absent" (ZeroB / ZeroC / ZeroD), as in ParODEmodule.c:240-290.  Unlike the
  * reference, the inputs are not echoed to stdout (:309-325), a malformed
  * argument raises ValueError instead of returning an unreferenced None
- * (:397-408), and tEnd is nudged by 1e-4 h so the truncating step count of
- * the solver (RungeKutta4thOrder.cpp:20) yields nSteps points (the SciLab
- * stub's rule, ParODEInterfaceStubs.sci:92).
+ * (:397-408).  tEnd = tStart + (nSteps - 1) tStep as the reference sets it
+ * (:330), so the solver's truncating step count (RungeKutta4thOrder.cpp:20)
+ * returns the same number of points as the reference's module: nSteps, or
+ * nSteps - 1 where the floating-point quotient falls just below nSteps - 1.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -189,7 +190,7 @@ static PyObject* ParODE_SimulateODE(PyObject* self, PyObject* args)
         PyErr_SetString(PyExc_ValueError, "nSteps must be positive");
         goto done;
     }
-    tEnd = tStart + (nSteps - 1) * tStep + 1e-4 * tStep;
+    tEnd = tStart + (nSteps - 1) * tStep;
     tVect = (float*)malloc(sizeof(float) * nSteps);
     xVect = (float*)malloc(sizeof(float) * (size_t)nSteps * nOutputs);
     Py_BEGIN_ALLOW_THREADS

@@ -82,7 +82,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
     phases_.resize(kOut + 1);
     auto stage_of = [&](int ph) {
         if (ph == kIn) return kStageInputs;
-        if (ph < kPost || ph == kPost) return ph == kPost ? kStagePowers : kStageMap;
+        if (ph <= kPost) return kStageMap;
         if (ph < kTables) return kStagePowers;
         if (ph == kTables) return kStageTables;
         if (ph == kStart) return kStageStartup;
@@ -146,7 +146,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
                 k::g::add_transpose(phases_[kPost], n, 5 * m, c.Bbar64_.get(), c.BbT64_.get());
         }
         // the FP64 power table P_{l+1} = P_l^2 (transposed)
-        if (!k::g::add_powers_small(phases_[kPost], na, L, PT))
+        if (!k::g::add_powers_small(phases_[kPow0], na, L, PT))
             for (int l = 0; l < L; ++l)
                 k::g::add_gemm(phases_[kPow0 + l], na, na, na, PT + l * na2, PT + l * na2,
                                PT + (l + 1) * na2);
@@ -161,7 +161,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         float* yScan = q.Y;
         if (abm) {
             k::g::add_startup(phases_[kStart], n, m, q.h, q.A, q.B, q.x0, q.U, c.seed());
-            k::g::add_start_outputs(phases_[kLeaves], n, m, p, pp, q.C != nullptr,
+            k::g::add_start_outputs(phases_[kOut], n, m, p, pp, q.C != nullptr,
                                     q.D != nullptr, c.CT64_.get(), c.DT64_.get(), c.seed(), q.U,
                                     q.Y);
             yScan = q.Y + 3L * p;
@@ -250,7 +250,12 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         for (auto& kv : phases_[pi]) {
             const k::g::Launch& Lr = kv.second;
             if (Lr.nsys() <= 0 || Lr.blocks() == 0) continue;
-            Ref r{&Lr, align(off), 0, stage_of((int)pi)};
+            // streams: the map on the main stream; the powers, level tables and
+            // ABM start-up on the side stream, beside the leaves and the first
+            // chunk pass; the rest after both
+            const int pos = (int)pi;
+            const int role = pos <= kPost ? 0 : pos <= kStart ? 1 : pos <= kChunk1 ? 2 : 3;
+            Ref r{&Lr, align(off), 0, stage_of(pos), role};
             r.start = align(r.args + Lr.args.size());
             off = r.start + Lr.start.size() * sizeof(unsigned);
             refs_.push_back(r);
@@ -271,18 +276,38 @@ const char* BatchSolve::stage_name(int stage)
 }
 
 void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
-                     const hipEvent_t* marks) const
+                     const hipEvent_t* marks, const ScanCore::SideStream* side) const
 {
     if (upload && !blob_.empty()) {
         std::memcpy(hostStage, blob_.data(), blob_.size());
         PARODE_HIP(hipMemcpyAsync(devDesc, hostStage, blob_.size(), hipMemcpyHostToDevice, s));
     }
     const char* base = static_cast<const char*>(devDesc);
+    auto launch = [&](hipStream_t st, const Ref& r) {
+        k::g::run(st, *r.L, base + r.args, reinterpret_cast<const unsigned*>(base + r.start));
+    };
+    if (side && !marks) {
+        if (!side->stream || !side->fork || !side->join)
+            throw std::invalid_argument("BatchSolve::run: incomplete side stream");
+        for (const Ref& r : refs_)
+            if (r.role == 0) launch(s, r);
+        PARODE_HIP(hipEventRecord(side->fork, s));
+        PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+        for (const Ref& r : refs_)
+            if (r.role == 1) launch(side->stream, r);
+        PARODE_HIP(hipEventRecord(side->join, side->stream));
+        for (const Ref& r : refs_)
+            if (r.role == 2) launch(s, r);
+        PARODE_HIP(hipStreamWaitEvent(s, side->join, 0));
+        for (const Ref& r : refs_)
+            if (r.role == 3) launch(s, r);
+        return;
+    }
     int stage = 0;
     if (marks) PARODE_HIP(hipEventRecord(marks[0], s));
     for (const Ref& r : refs_) {
         while (marks && stage < r.stage) PARODE_HIP(hipEventRecord(marks[++stage], s));
-        k::g::run(s, *r.L, base + r.args, reinterpret_cast<const unsigned*>(base + r.start));
+        launch(s, r);
     }
     while (marks && stage < kStages) PARODE_HIP(hipEventRecord(marks[++stage], s));
 }
@@ -305,10 +330,14 @@ struct Group {
     void* stage = nullptr;
     bool uploaded = false;
     hipEvent_t marks[parode::BatchSolve::kStages + 1] = {};
+    parode::ScanCore::SideStream side;
     ~Group()
     {
         batch.reset();
         arena.reset();
+        if (side.fork) (void)hipEventDestroy(side.fork);
+        if (side.join) (void)hipEventDestroy(side.join);
+        if (side.stream) (void)hipStreamDestroy(side.stream);
         for (hipEvent_t e : marks)
             if (e) (void)hipEventDestroy(e);
         if (desc) (void)hipFree(desc);
@@ -385,7 +414,13 @@ int ParODE_GroupExecute(void* group, void* stream)
     if (!g) return ParODE_InvalidSystem;
     try {
         PARODE_HIP(hipSetDevice(g->device));
-        g->batch->run(static_cast<hipStream_t>(stream), g->stage, g->desc, !g->uploaded);
+        if (!g->side.stream) {
+            PARODE_HIP(hipStreamCreate(&g->side.stream));
+            PARODE_HIP(hipEventCreateWithFlags(&g->side.fork, hipEventDisableTiming));
+            PARODE_HIP(hipEventCreateWithFlags(&g->side.join, hipEventDisableTiming));
+        }
+        g->batch->run(static_cast<hipStream_t>(stream), g->stage, g->desc, !g->uploaded, nullptr,
+                      &g->side);
         g->uploaded = true;
         return ParODE_OK;
     } catch (std::exception& e) {

@@ -57,9 +57,14 @@ public:
     // devDesc (device) unless upload is false (they are there from an earlier
     // run), then queues every launch on s.  Both areas must stay untouched
     // until the work on s is done.  marks, when given, are kStages + 1 events
-    // recorded before each stage's launches and after the last.
+    // recorded before each stage's launches and after the last (every launch
+    // then runs on s, one after the other).  Else, with side, the FP64 power
+    // tables, the level tables and the ABM start-up run on side->stream
+    // (forked after the step map, joined before the first tree level) beside
+    // the leaves and the first chunk pass.
     void run(hipStream_t s, void* hostStage, void* devDesc, bool upload = true,
-             const hipEvent_t* marks = nullptr) const;
+             const hipEvent_t* marks = nullptr,
+             const ScanCore::SideStream* side = nullptr) const;
     int launches() const { return (int)refs_.size(); }
 
     // Stages of a grouped solve, in launch order (for timing).
@@ -81,6 +86,7 @@ private:
         const k::g::Launch* L;
         size_t args, start;
         int stage;
+        int role;    // 0 map, 1 side stream, 2 beside the side stream, 3 after the join
     };
     std::vector<Ref> refs_;
 };

@@ -232,13 +232,14 @@ void to_f64(hipStream_t s, long count, const T* in, double* out);
 // launches in key order.
 namespace g {
 enum Kind {
-    kPrep, kGemm16, kGemmKs2, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
+    kPrep, kGemm16, kGemmKs2, kGemmT64, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
     kSample, kStartup, kStartOut, kLeavesPlain, kLeavesFac, kLeavesSimt,
-    kChunksFac16, kChunksFac32, kChunksFac64, kChunksFac128,
+    kChunksFac16, kChunksFac32, kChunksFac48, kChunksFac64, kChunksFac96, kChunksFac128,
     kChunksTwo160, kChunksTwo192, kChunksTwo224, kChunksTwo256,
-    kDense16, kDense32, kDense64, kDense128, kDense256,
+    kDense16, kDense32, kDense48, kDense64, kDense96, kDense128, kDense160, kDense192, kDense224,
+    kDense256,
     kLevel32, kLevel32Delta, kLevel64, kGather, kTopMfma, kTopSimt, kRoot,
-    kOutputsLanes, kOutputsKc8, kOutputsKc16, kOutputsKc24, kOutputsKc32
+    kOutputsLanes, kOutputsKc8, kOutputsKc16, kOutputsKc24, kOutputsKc32, kOutputsMfma
 };
 struct Launch {
     int kind = -1;
@@ -250,6 +251,7 @@ struct Launch {
     int nsys() const { return (int)start.size() - 1; }
     unsigned blocks() const { return start.back(); }
 };
+// key: (kind, flag + 2 x LDS size class)
 using Phase = std::map<std::pair<int, int>, Launch>;
 
 // Each add_* appends one system's launch of a kernel to a phase (its kind and

@@ -235,6 +235,87 @@ __global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int
     d_gemm_f64_ks<KSPLIT>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z);
 }
 
+// FP64 GEMM on 64 x 64 tiles of Z for the squarings of the power tables: four
+// waves, each a 32 x 32 quadrant as 2 x 2 tiles of v_mfma_f64_16x16x4f64 (four
+// independent accumulators, each operand fragment used twice), K in slices of
+// 16 staged in LDS (odd pitches), double buffered: the next slice's loads are
+// in flight while the current one is multiplied, one barrier per slice.  Twice
+// the arithmetic per byte of operands of the 32 x 32 tiles.
+constexpr int kG64 = 64, kG64KB = 16;
+
+__device__ __forceinline__ void d_gemm_f64_t64(const VBlock vb, int r, int kk, int c,
+                                               const double* __restrict__ X,
+                                               const double* __restrict__ Y,
+                                               double* __restrict__ Z)
+{
+    __shared__ double As[2][kG64][kG64KB + 1];
+    __shared__ double Bs[2][kG64KB][kG64 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1, l16 = lane & 15, q4 = lane >> 4;
+    const int i0 = (int)vb.y * kG64, j0 = (int)vb.x * kG64;
+    const int ar = tid >> 2, ak = (tid & 3) * 4;     // A slice: row, first of four k
+    const int bk = tid >> 4, bj = (tid & 15) * 4;    // B slice: k, first of four columns
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gi = i0 + ar, gk = k0 + ak + q;
+            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
+            const int gk2 = k0 + bk, gj = j0 + bj + q;
+            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            As[buf][ar][ak + q] = ra[q];
+            Bs[buf][bk][bj + q] = rb[q];
+        }
+    };
+    doublex4 acc[2][2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = doublex4{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < kk; k0 += kG64KB) {
+        const bool more = k0 + kG64KB < kk;
+        if (more) load(k0 + kG64KB);
+#pragma unroll
+        for (int ks = 0; ks < kG64KB; ks += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = As[buf][wr * 32 + t * 16 + l16][ks + q4];
+                b[t] = Bs[buf][ks + q4][wc * 32 + t * 16 + l16];
+            }
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj],
+                                                                      0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // D[i][j] is in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wr * 32 + ti * 16 + 4 * v + q4;
+                const int j = j0 + wc * 32 + tj * 16 + l16;
+                if (i < r && j < c) Z[(long)i * c + j] = acc[ti][tj][v];
+            }
+}
+
 void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double* Y, double* Z,
               bool mfma)
 {
@@ -875,7 +956,7 @@ template <typename T, typename TU, bool FAC>
 __device__ __forceinline__ void d_leaves_mfma(const VBlock vb, int n, int m, int kq, int na, int off,
                                                      long nReal, long nPad, int groups,
                                                      int pitch, unsigned magic, int kg,
-                                                     LeafComb G, bool writeZeros, int zlo,
+                                                     const LeafComb& G, bool writeZeros, int zlo,
                                                      const double* __restrict__ BbT,
                                                      const TU* __restrict__ U, long sL, long oL,
                                                      long rs, T* __restrict__ X)
@@ -959,13 +1040,18 @@ __device__ __forceinline__ void d_leaves_mfma(const VBlock vb, int n, int m, int
         // rows from zlo on, rows [zlo, off) of real elements and every zero row
         // of padding elements
         if (nz > 0) {
-            const int zfirst = (writeZeros || zlo < off) ? 0 : creal;
-            for (int le = zfirst; le < cnt; ++le) {
-                T* xe = X + (eb + le) * na;
-                const bool all = writeZeros || le >= creal;
-                const int zb = all ? 0 : zlo, ze = all ? nz : off;
-                for (int z = zb + (int)threadIdx.x; z < ze; z += 256)
-                    xe[z < off ? z : z + n] = (T)0;
+            // (leaf, row) pairs spread over the work-group: rows [zlo, off) of
+            // the real leaves [partFrom, fullFrom), every zero row of [fullFrom, cnt)
+            const int fullFrom = writeZeros ? 0 : creal;
+            const int partFrom = (!writeZeros && zlo < off) ? 0 : fullFrom;
+            const int w = off - zlo;
+            for (int t = threadIdx.x; t < (fullFrom - partFrom) * w; t += 256) {
+                const int q = t / w;
+                X[(eb + partFrom + q) * na + zlo + (t - q * w)] = (T)0;
+            }
+            for (int t = threadIdx.x; t < (cnt - fullFrom) * nz; t += 256) {
+                const int q = t / nz, z = t - q * nz;
+                X[(eb + fullFrom + q) * na + (z < off ? z : z + n)] = (T)0;
             }
         }
         __syncthreads();
@@ -2156,8 +2242,12 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
         launch(std::integral_constant<int, 16>());
     else if (n <= 32)
         launch(std::integral_constant<int, 32>());
+    else if (n <= 48)
+        launch(std::integral_constant<int, 48>());
     else if (n <= 64)
         launch(std::integral_constant<int, 64>());
+    else if (n <= 96)
+        launch(std::integral_constant<int, 96>());
     else if (n <= 128)
         launch(std::integral_constant<int, 128>());
     else if (n <= 160)
@@ -2317,10 +2407,20 @@ void chunks_dense(hipStream_t s, bool pass2, int na, int lc, long S, long nReal,
         launch(std::integral_constant<int, 16>());
     else if (na <= 32)
         launch(std::integral_constant<int, 32>());
+    else if (na <= 48)
+        launch(std::integral_constant<int, 48>());
     else if (na <= 64)
         launch(std::integral_constant<int, 64>());
+    else if (na <= 96)
+        launch(std::integral_constant<int, 96>());
     else if (na <= 128)
         launch(std::integral_constant<int, 128>());
+    else if (na <= 160)
+        launch(std::integral_constant<int, 160>());
+    else if (na <= 192)
+        launch(std::integral_constant<int, 192>());
+    else if (na <= 224)
+        launch(std::integral_constant<int, 224>());
     else
         launch(std::integral_constant<int, 256>());
     check_launch("chunks_dense");
@@ -3130,6 +3230,106 @@ __global__ __launch_bounds__(kOutLanes) void k_outputs_kc(int n, int m, int p, i
     d_outputs_kc<TX, TU, JG>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
 }
 
+// The same outputs on the FP64 matrix cores (v_mfma_f64_16x16x4f64), as the
+// product of the elements' state blocks (FP32, converted exactly) and inputs
+// with C^T and D^T: a work-group of four waves takes 64 elements, whose state
+// blocks go through LDS in chunks of kOutKcM states (coalesced rows); each wave
+// forms 16 of the elements times up to kOutMfmaTiles 16-output tiles, K over
+// the n states, then the m inputs (read directly, FP64).  Fragments:
+//   A (16 elements x 4 k): lane l holds element e0 + l%16, k = k0 + l/16;
+//   B (4 k x 16 outputs):  lane l holds CT[k0 + l/16][j0 + l%16];
+//   D: lane l, register v holds element e0 + 4v + l/16, output j0 + l%16.
+// Output columns past pp (the padding of C^T and D^T) and k past n or m are
+// read clamped (their A values are zero); elements past N repeat the last one
+// and are not stored.
+constexpr int kOutMfmaTiles = 4, kOutKcM = 32;
+
+__device__ __forceinline__ void d_outputs_mfma(const VBlock vb, int n, int m, int p, int pp,
+                                               int na, int oX, bool useC, bool useD, long N,
+                                               const double* __restrict__ CT,
+                                               const double* __restrict__ DT,
+                                               const float* __restrict__ X,
+                                               const double* __restrict__ U, long sO, long oO,
+                                               float* __restrict__ Y)
+{
+    __shared__ float xs[64][kOutKcM + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, k4 = lane >> 4;
+    const int jtiles = (p + 15) / 16;
+    const int sel = tid >> 2, sb = (tid & 3) * 8;   // staging: element, first of 8 states
+    for (long eb = vb.x * 64; eb < N; eb += vb.gx * 64) {
+        const int cnt = N - eb < 64 ? (int)(N - eb) : 64;
+        const long e0 = eb + 16 * wave;
+        const long ea = eb + (16 * wave + l16 < cnt ? 16 * wave + l16 : cnt - 1);
+        const float* xstage = X + (eb + (sel < cnt ? sel : cnt - 1)) * na + (long)oX * n;
+        const double* ur = U + (ea * sO + oO) * m;
+        for (int jt0 = 0; jt0 < jtiles; jt0 += kOutMfmaTiles) {
+            const int nt = jtiles - jt0 < kOutMfmaTiles ? jtiles - jt0 : kOutMfmaTiles;
+            doublex4 acc[kOutMfmaTiles];
+            int jb[kOutMfmaTiles];
+#pragma unroll
+            for (int t = 0; t < kOutMfmaTiles; ++t) {
+                acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+                const int j = (jt0 + t) * 16 + l16;
+                jb[t] = j < pp ? j : pp - 1;
+            }
+            if (useC) {
+                for (int kc0 = 0; kc0 < n; kc0 += kOutKcM) {
+                    float v[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int k = kc0 + sb + q;
+                        v[q] = k < n ? xstage[k] : 0.0f;
+                    }
+                    __syncthreads();                  // the previous chunk has been read
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) xs[sel][sb + q] = v[q];
+                    __syncthreads();
+                    const int kn = n - kc0 < kOutKcM ? n - kc0 : kOutKcM;
+                    for (int ks = 0; ks < kn; ks += 4) {
+                        const double a = (double)xs[16 * wave + l16][ks + k4];
+                        const int kc = kc0 + ks + k4 < n ? kc0 + ks + k4 : n - 1;
+                        double b[kOutMfmaTiles];
+#pragma unroll
+                        for (int t = 0; t < kOutMfmaTiles; ++t)
+                            b[t] = t < nt ? CT[(long)kc * pp + jb[t]] : 0.0;
+#pragma unroll
+                        for (int t = 0; t < kOutMfmaTiles; ++t)
+                            if (t < nt)
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0,
+                                                                             0, 0);
+                    }
+                }
+            }
+            if (useD) {
+                for (int k0 = 0; k0 < m; k0 += 4) {
+                    const int k = k0 + k4, kc = k < m ? k : m - 1;
+                    const double a = k < m ? ur[kc] : 0.0;
+                    double b[kOutMfmaTiles];
+#pragma unroll
+                    for (int t = 0; t < kOutMfmaTiles; ++t)
+                        b[t] = t < nt ? DT[(long)kc * pp + jb[t]] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < kOutMfmaTiles; ++t)
+                        if (t < nt)
+                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0,
+                                                                         0);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kOutMfmaTiles; ++t) {
+                const int j = (jt0 + t) * 16 + l16;
+                if (t >= nt || j >= p) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const long e = e0 + 4 * v + k4;
+                    if (e < N) Y[e * p + j] = (float)acc[t][v];
+                }
+            }
+        }
+    }
+}
+
 __global__ void k_pad_cast(int rows, int cols, int pp, const float* __restrict__ in,
                            double* __restrict__ out)
 {
@@ -3628,6 +3828,13 @@ __global__ __launch_bounds__(256) void kg_gemm16(const Gemm* __restrict__ args,
     d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
 }
 
+__global__ __launch_bounds__(256) void kg_gemm_t64(const Gemm* __restrict__ args,
+                                                   const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Gemm);
+    d_gemm_f64_t64(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+}
+
 __global__ __launch_bounds__(512) void kg_gemm_ks2(const Gemm* __restrict__ args,
                                                    const unsigned* __restrict__ st, int ns)
 {
@@ -3793,6 +4000,14 @@ __global__ __launch_bounds__(kOutLanes) void kg_out_lanes(const Outputs* __restr
                                    a.DT, a.X, a.U, a.sO, a.oO, a.Y);
 }
 
+__global__ __launch_bounds__(256) void kg_out_mfma(const Outputs* __restrict__ args,
+                                                   const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Outputs);
+    d_outputs_mfma(vb, a.n, a.m, a.p, a.pp, a.na, a.oX, a.useC, a.useD, a.N, a.CT, a.DT, a.X,
+                   a.U, a.sO, a.oO, a.Y);
+}
+
 template <int JG>
 __global__ __launch_bounds__(kOutLanes) void kg_out_kc(const Outputs* __restrict__ args,
                                                        const unsigned* __restrict__ st, int ns)
@@ -3806,9 +4021,9 @@ __global__ __launch_bounds__(kOutLanes) void kg_out_kc(const Outputs* __restrict
 
 // The record of the launch (kind, flag) in phase ph, created on first use.
 template <typename A>
-Launch& slot(Phase& ph, int kind, bool flag = false)
+Launch& slot(Phase& ph, int kind, bool flag, int cls)
 {
-    Launch& L = ph[{kind, flag ? 1 : 0}];
+    Launch& L = ph[{kind, (flag ? 1 : 0) + 2 * cls}];
     if (L.kind < 0) {
         L.kind = kind;
         L.flag = flag;
@@ -3818,11 +4033,21 @@ Launch& slot(Phase& ph, int kind, bool flag = false)
     return L;
 }
 
+// Systems whose dynamic LDS differs go to separate launches by size class
+// (up to 8, 16, 32, 48, 64 KiB ...): a launch reserves its largest system's
+// LDS for every work-group, which would cap the occupancy of the others.
+int lds_class(size_t lds)
+{
+    if (lds == 0) return 0;
+    const size_t kib = (lds + 1023) / 1024;
+    return kib <= 8 ? 1 : kib <= 16 ? 2 : kib <= 32 ? 3 : kib <= 48 ? 4 : 5 + (int)((kib - 49) / 16);
+}
+
 template <typename A>
 void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds = 0)
 {
     if (blocks < 1) return;
-    Launch& L = slot<A>(ph, kind, flag);
+    Launch& L = slot<A>(ph, kind, flag, lds_class(lds));
     const size_t o = L.args.size();
     L.args.resize(o + sizeof(A));
     std::memcpy(L.args.data() + o, &a, sizeof(A));
@@ -3837,14 +4062,14 @@ void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds 
 void add_cast(Phase& ph, long count, const float* in, double* out)
 {
     if (count <= 0) return;
-    const long b = blocks_for(count, 256, 4096);
+    const long b = blocks_for(count, 256, 512);
     append(ph, kPrep, false, Prep{0, 0, 0, 0, count, 0.0, in, out, b}, b);
 }
 
 void add_fill(Phase& ph, long count, double v, double* out)
 {
     if (count <= 0) return;
-    const long b = blocks_for(count, 256, 4096);
+    const long b = blocks_for(count, 256, 512);
     append(ph, kPrep, false, Prep{1, 0, 0, 0, count, v, nullptr, out, b}, b);
 }
 
@@ -3858,7 +4083,10 @@ void add_transpose_pad(Phase& ph, int rows, int cols, int pp, const float* in, d
 void add_gemm(Phase& ph, int r, int kk, int c, const double* X, const double* Y, double* Z)
 {
     if (r <= 0 || c <= 0) return;
-    if (r >= 320 && c >= 320 && kk >= 64) {
+    if (r >= 256 && c >= 256 && kk >= 64) {   // the squarings of wide power tables
+        const long gx = (c + kG64 - 1) / kG64, gy = (r + kG64 - 1) / kG64;
+        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy);
+    } else if (r >= 320 && c >= 320 && kk >= 64) {
         const long gx = (c + kGemmKsT - 1) / kGemmKsT, gy = (r + kGemmKsT - 1) / kGemmKsT;
         const size_t stage = (size_t)(kGemmKsT * (kGemmKsKB + 1) + kGemmKsKB * (kGemmKsT + 1));
         append(ph, kGemmKs2, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy,
@@ -3888,7 +4116,7 @@ void add_abm_combine(Phase& ph, int n, int m, double h, const double* A, const d
 void add_transpose(Phase& ph, int rows, int cols, const double* in, double* out)
 {
     if ((long)rows * cols <= 0) return;
-    const long b = blocks_for((long)rows * cols, 256, 4096);
+    const long b = blocks_for((long)rows * cols, 256, 512);
     append(ph, kTranspose, false, Transpose{rows, cols, in, out, b}, b);
 }
 
@@ -3907,7 +4135,7 @@ void add_tables(Phase& ph, int na, int nap, int nrp, int nlev, int deltaBelow, c
 {
     if (nlev <= 0) return;
     if (!PTm) nap = nrp = na;
-    const long b = blocks_for((long)nrp * nap * nlev, 256, 16384);
+    const long b = blocks_for((long)nrp * nap * nlev, 256, 512);
     append(ph, kTables, false, Tables{na, nap, nrp, nlev, deltaBelow, PT64, PT32, PTm, b}, b);
 }
 
@@ -3915,7 +4143,7 @@ void add_sample(Phase& ph, int m, const double* coef, double t0, double dt, long
                 uint64_t seed, double* out)
 {
     if (S * m <= 0) return;
-    const long b = blocks_for(S * m, 256, 8192);
+    const long b = blocks_for(S * m, 256, 256);
     append(ph, kSample, false, Sample{m, coef, t0, dt, S, sigma, seed, out, b}, b);
 }
 
@@ -3967,10 +4195,15 @@ bool add_leaves(Phase& ph, int n, int m, int kq, int kg, int na, int off, long n
     if (lds > 65536 || m >= 65536 || lds / sizeof(double) >= 65536 || perBlock * m >= 65536)
         return false;
     const unsigned magic = (unsigned)(((1ULL << 32) + m - 1) / m);
-    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 2048), 1);
+    // persistent work-groups: a few hundred per system (each walks its blocks,
+    // the next block's samples loaded during the current one), so that the
+    // per-work-group start-up is paid rarely
+    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 256), 1);
+    // without writeZeros the first reader (a chunk pass) reads only rows
+    // [off, off + n): no other row is written for real leaves
     append(ph, kg > 0 ? kLeavesFac : kLeavesPlain, false,
            Leaves{n, m, kq, na, off, nReal, nPad, groups, pitch, magic, kg, G, writeZeros,
-                  off / 32 * 32, Bf, U, sL, oL, rs, X, b},
+                  writeZeros ? off / 32 * 32 : off, Bf, U, sL, oL, rs, X, b},
            b, lds);
     return true;
 }
@@ -3993,10 +4226,10 @@ bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nRea
     while (groups > 1 && lds_for(groups) > 65536) groups >>= 1;
     if (lds_for(groups) > 65536) return false;
     const long perBlock = (long)groups * kLeafEB;
-    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 1L << 16), 1);
+    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 256), 1);
     append(ph, kLeavesSimt, false,
            Leaves{n, m, kq, na, off, nReal, nPad, groups, tpeLog, 0u, 0, LeafComb{}, writeZeros,
-                  off / 32 * 32, BbT, U, sL, oL, rs, X, b},
+                  writeZeros ? off / 32 * 32 : off, BbT, U, sL, oL, rs, X, b},
            b, lds_for(groups));
     return true;
 }
@@ -4009,14 +4242,17 @@ bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad,
         return false;
     const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
     const Chunks a{n, lc, nReal, A64, A264, abm_chunk_factors(h), X, b};
-    const int np = n <= 16 ? 16 : n <= 32 ? 32 : n <= 64 ? 64 : n <= 128 ? 128 : n <= 160 ? 160
-                 : n <= 192 ? 192 : n <= 224 ? 224 : 256;
-    const int kind = np == 16 ? kChunksFac16 : np == 32 ? kChunksFac32 : np == 64 ? kChunksFac64
-                   : np == 128 ? kChunksFac128 : np == 160 ? kChunksTwo160
-                   : np == 192 ? kChunksTwo192 : np == 224 ? kChunksTwo224 : kChunksTwo256;
+    // padded state sizes: factored up to 128, two products above
+    static const int kNp[] = {16, 32, 48, 64, 96, 128, 160, 192, 224, 256};
+    static const int kKind[] = {kChunksFac16, kChunksFac32, kChunksFac48, kChunksFac64,
+                                kChunksFac96, kChunksFac128, kChunksTwo160, kChunksTwo192,
+                                kChunksTwo224, kChunksTwo256};
+    int t = 0;
+    while (kNp[t] < n) ++t;
+    const int np = kNp[t];
     const size_t lds = np <= 128 ? (size_t)2 * kChunkCols * (2 * np + 4) * sizeof(float)
                                  : (size_t)2 * kChunkCols * (np + 4) * sizeof(float);
-    append(ph, kind, pass2, a, b, lds);
+    append(ph, kKind[t], pass2, a, b, lds);
     return true;
 }
 
@@ -4026,11 +4262,13 @@ bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nP
     if (!chunks_dense_supported(na) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
         return false;
     const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
-    const int np = na <= 16 ? 16 : na <= 32 ? 32 : na <= 64 ? 64 : na <= 128 ? 128 : 256;
-    const int kind = np == 16 ? kDense16 : np == 32 ? kDense32 : np == 64 ? kDense64
-                   : np == 128 ? kDense128 : kDense256;
-    append(ph, kind, pass2, Chunks{na, lc, nReal, P64, nullptr, AbmFactors{}, X, b}, b,
-           (size_t)kChunkCols * (np + 4) * sizeof(float));
+    static const int kNp[] = {16, 32, 48, 64, 96, 128, 160, 192, 224, 256};
+    static const int kKind[] = {kDense16, kDense32, kDense48, kDense64, kDense96, kDense128,
+                                kDense160, kDense192, kDense224, kDense256};
+    int t = 0;
+    while (kNp[t] < na) ++t;
+    append(ph, kKind[t], pass2, Chunks{na, lc, nReal, P64, nullptr, AbmFactors{}, X, b}, b,
+           (size_t)kChunkCols * (kNp[t] + 4) * sizeof(float));
     return true;
 }
 
@@ -4049,7 +4287,7 @@ void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long
 void add_gather(Phase& ph, int na, long d, long np, const float* X, float* S)
 {
     if (np <= 0) return;
-    const long b = blocks_for(np * na, 256, 4096);
+    const long b = blocks_for(np * na, 256, 1024);
     append(ph, kGather, false, Gather{na, d, np, X, S, b}, b);
 }
 
@@ -4078,9 +4316,14 @@ bool add_outputs(Phase& ph, int n, int m, int p, int pp, int na, int oX, bool us
 {
     if (N <= 0) return true;
     if (p <= 0 || pp % 8 != 0 || pp < p) return false;
-    const long b = std::max<long>(std::min<long>((N + kOutLanes - 1) / kOutLanes, 4096), 1);
+    const long b = std::max<long>(std::min<long>((N + kOutLanes - 1) / kOutLanes, 1024), 1);
     const Outputs a{n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y, b};
-    if (n <= kOutLanes) {
+    if (n >= 32) {   // wide states: the FP64 matrix cores
+        const long bm = std::max<long>(std::min<long>((N + 63) / 64, 512), 1);
+        Outputs am = a;
+        am.gx = bm;
+        append(ph, kOutputsMfma, false, am, bm);
+    } else if (n <= kOutLanes) {
         append(ph, kOutputsLanes, false, a, b, (size_t)kOutLanes * (n + 1) * sizeof(float));
     } else {
         const int kind = pp % 32 == 0 ? kOutputsKc32 : pp % 24 == 0 ? kOutputsKc24
@@ -4111,6 +4354,7 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kPrep: PARODE_RUN(kg_prep, Prep, 256); break;
     case kGemm16: PARODE_RUN(kg_gemm16, Gemm, 256); break;
     case kGemmKs2: PARODE_RUN(kg_gemm_ks2, Gemm, 512); break;
+    case kGemmT64: PARODE_RUN(kg_gemm_t64, Gemm, 256); break;
     case kRk4Combine: PARODE_RUN(kg_rk4_combine, Rk4Comb, 256); break;
     case kAbmCombine: PARODE_RUN(kg_abm_combine, AbmComb, 256); break;
     case kTranspose: PARODE_RUN(kg_transpose, Transpose, 256); break;
@@ -4124,7 +4368,9 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kLeavesSimt: PARODE_RUN(kg_leaves_simt, Leaves, 256); break;
     case kChunksFac16: PARODE_RUN2((kg_chunks_fac<16, false>), (kg_chunks_fac<16, true>), Chunks, 64); break;
     case kChunksFac32: PARODE_RUN2((kg_chunks_fac<32, false>), (kg_chunks_fac<32, true>), Chunks, 128); break;
+    case kChunksFac48: PARODE_RUN2((kg_chunks_fac<48, false>), (kg_chunks_fac<48, true>), Chunks, 192); break;
     case kChunksFac64: PARODE_RUN2((kg_chunks_fac<64, false>), (kg_chunks_fac<64, true>), Chunks, 256); break;
+    case kChunksFac96: PARODE_RUN2((kg_chunks_fac<96, false>), (kg_chunks_fac<96, true>), Chunks, 384); break;
     case kChunksFac128: PARODE_RUN2((kg_chunks_fac<128, false>), (kg_chunks_fac<128, true>), Chunks, 512); break;
     case kChunksTwo160: PARODE_RUN2((kg_chunks_two<160, false>), (kg_chunks_two<160, true>), Chunks, 320); break;
     case kChunksTwo192: PARODE_RUN2((kg_chunks_two<192, false>), (kg_chunks_two<192, true>), Chunks, 384); break;
@@ -4132,8 +4378,13 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kChunksTwo256: PARODE_RUN2((kg_chunks_two<256, false>), (kg_chunks_two<256, true>), Chunks, 512); break;
     case kDense16: PARODE_RUN2((kg_chunks_dense<16, false>), (kg_chunks_dense<16, true>), Chunks, 64); break;
     case kDense32: PARODE_RUN2((kg_chunks_dense<32, false>), (kg_chunks_dense<32, true>), Chunks, 128); break;
+    case kDense48: PARODE_RUN2((kg_chunks_dense<48, false>), (kg_chunks_dense<48, true>), Chunks, 192); break;
     case kDense64: PARODE_RUN2((kg_chunks_dense<64, false>), (kg_chunks_dense<64, true>), Chunks, 256); break;
+    case kDense96: PARODE_RUN2((kg_chunks_dense<96, false>), (kg_chunks_dense<96, true>), Chunks, 384); break;
     case kDense128: PARODE_RUN2((kg_chunks_dense<128, false>), (kg_chunks_dense<128, true>), Chunks, 512); break;
+    case kDense160: PARODE_RUN2((kg_chunks_dense<160, false>), (kg_chunks_dense<160, true>), Chunks, 640); break;
+    case kDense192: PARODE_RUN2((kg_chunks_dense<192, false>), (kg_chunks_dense<192, true>), Chunks, 768); break;
+    case kDense224: PARODE_RUN2((kg_chunks_dense<224, false>), (kg_chunks_dense<224, true>), Chunks, 896); break;
     case kDense256: PARODE_RUN2((kg_chunks_dense<256, false>), (kg_chunks_dense<256, true>), Chunks, 1024); break;
     case kLevel32: PARODE_RUN2((kg_level<float, false, false>), (kg_level<float, true, false>), Level, 256); break;
     case kLevel32Delta: PARODE_RUN2((kg_level<float, false, true>), (kg_level<float, true, true>), Level, 256); break;
@@ -4143,6 +4394,7 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kTopSimt: PARODE_RUN(kg_top<false>, Top, kTopThreads); break;
     case kRoot: PARODE_RUN(kg_root, Root, 256); break;
     case kOutputsLanes: PARODE_RUN(kg_out_lanes, Outputs, kOutLanes); break;
+    case kOutputsMfma: PARODE_RUN(kg_out_mfma, Outputs, 256); break;
     case kOutputsKc8: PARODE_RUN(kg_out_kc<8>, Outputs, kOutLanes); break;
     case kOutputsKc16: PARODE_RUN(kg_out_kc<16>, Outputs, kOutLanes); break;
     case kOutputsKc24: PARODE_RUN(kg_out_kc<24>, Outputs, kOutLanes); break;

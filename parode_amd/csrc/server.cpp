@@ -123,6 +123,12 @@ Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
     lanes_.resize(lanes);
     load_.assign(lanes, 0);
     for (auto& s : lanes_) PARODE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    sides_.resize(lanes);
+    for (auto& sd : sides_) {
+        PARODE_HIP(hipStreamCreateWithFlags(&sd.stream, hipStreamNonBlocking));
+        PARODE_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+        PARODE_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+    }
     st_.arenaBytes = arena_->capacity();
     worker_ = std::thread([this] { loop(); });
 }
@@ -136,6 +142,11 @@ Server::~Server()
     work_cv_.notify_all();
     if (worker_.joinable()) worker_.join();
     for (auto& s : lanes_) (void)hipStreamDestroy(s);
+    for (auto& sd : sides_) {
+        if (sd.fork) (void)hipEventDestroy(sd.fork);
+        if (sd.join) (void)hipEventDestroy(sd.join);
+        if (sd.stream) (void)hipStreamDestroy(sd.stream);
+    }
 }
 
 long Server::submit(const ServeRequest& r0)
@@ -298,7 +309,7 @@ bool Server::start(Group& g)
     check_range("launch records", upTot + yTot, descBytes, stage->bytes);
     PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
     if (g.grouped) {
-        g.batch->run(s, stage->p + upTot + yTot, g.desc);
+        g.batch->run(s, stage->p + upTot + yTot, g.desc, true, nullptr, &sides_[g.lane]);
     } else {
         const Job& j = *g.jobs[0];
         const ServeRequest& r = j.r;

@@ -83,6 +83,7 @@ private:
     int device_ = 0, precision_ = 0, depth_ = 2;
     size_t groupCap_ = 1024;   // requests per group (halved while memory is short)
     std::vector<hipStream_t> lanes_;
+    std::vector<ScanCore::SideStream> sides_;   // per lane: the grouped solves' side stream
     std::vector<int> load_;
     std::unique_ptr<Arena> arena_, pinned_;
     std::mutex mu_;

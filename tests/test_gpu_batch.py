@@ -1,7 +1,7 @@
 """Grouped launches of the served path (BatchSolve): many systems of uneven
 state size and horizon run each stage of the scan in one launch per kernel.
 Every padded-size variant of the chunk recurrences (RK4 dense 16..256, ABM
-factored 16..128 and two-product 160..256), short horizons without chunks,
+factored 16..128 and two-product 160..256, in steps of 16 or 32), short horizons without chunks,
 tiny systems whose whole tree runs in one work-group, and scan sizes below 16
 (FP64 tree levels, separate copy buffers) are checked against the FP64
 sequential recurrence at the mixed-precision bar, in one group; each result
@@ -40,6 +40,12 @@ SHAPES = [
     (1, 250, 30, 30, 6000),   # two-product, 256
     (0, 64, 8, 8, 2000),      # short horizon: no chunks, K-split tree from level 0
     (1, 20, 2, 2, 3000),
+    (0, 45, 5, 5, 20000),     # dense chunks, 48 padded states
+    (0, 90, 11, 11, 20000),   # dense 96
+    (0, 150, 18, 18, 12000),  # dense 160
+    (0, 180, 22, 22, 10000),  # dense 192
+    (1, 40, 5, 5, 20000),     # factored 48
+    (1, 80, 10, 10, 12000),   # factored 96
 ]
 
 
@@ -94,7 +100,7 @@ def test_grouped_variants_meet_the_fp64_bar_and_equal_alone():
         assert err <= MIXED_BAR, (s.method, s.n, s.steps, err)
     print(f"worst {worst:.3e}")
     # a request alone (a group of one) gives the same bits as in the group
-    for k in (1, 6, 9, 13, 15):
+    for k in (1, 6, 9, 13, 15, 20, 23):
         alone, _ = _serve([systems[k]], lanes=1)
         np.testing.assert_array_equal(alone[0], ys[k], err_msg=f"shape {SHAPES[k]}")
 
