@@ -3959,14 +3959,14 @@ __global__ __launch_bounds__(NP * 4) void kg_chunks_dense(const Chunks* __restri
 
 // FP32 levels on 16 x 16 tiles of the padded MFMA table (TT = float), FP64
 // levels on 16 x 16 tiles of the transposed FP64 power (TT = double).
-template <typename TT, bool DOWN, bool DELTA>
+template <typename TT, bool DOWN, bool DELTA, bool SMALL>
 __global__ __launch_bounds__(256) void kg_level(const Level* __restrict__ args,
                                                 const unsigned* __restrict__ st, int ns)
 {
     PARODE_GROUP(Level);
-    d_level_ks<TT, float, DOWN, DELTA, sizeof(TT) == 4>(
-        vb, a.na, a.ldp, a.d, a.np, static_cast<const TT*>(a.PT), a.S, a.X, DOWN ? a.Sn : nullptr,
-        DOWN ? a.npn : 0L);
+    d_level_ks<TT, float, DOWN, DELTA, SMALL>(vb, a.na, a.ldp, a.d, a.np,
+                                              static_cast<const TT*>(a.PT), a.S, a.X,
+                                              DOWN ? a.Sn : nullptr, DOWN ? a.npn : 0L);
 }
 
 __global__ __launch_bounds__(256) void kg_gather(const Gather* __restrict__ args,
@@ -4279,8 +4279,13 @@ void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long
     if (!f64 && ldp % 32 != 0) throw std::invalid_argument("add_level: FP32 table must be padded");
     if (down && (!S || (Sn && (Sn == S || npn < 0 || npn > 2 * np))))
         throw std::invalid_argument("add_level: bad copy buffers");
-    const long gx = (np + 15) / 16, gy = (na + 15) / 16;
-    const int kind = f64 ? kLevel64 : delta ? kLevel32Delta : kLevel32;
+    // FP32: 32 x 32 tiles (half the table and state traffic per product) once
+    // the system's own grid of them has 64 work-groups, else 16 x 16
+    const bool wide = !f64 && ((np + 31) / 32) * ((na + 31) / 32) >= 64;
+    const int T = wide ? 32 : 16;
+    const long gx = (np + T - 1) / T, gy = (na + T - 1) / T;
+    const int kind = f64 ? kLevel64 : delta ? (wide ? kLevel32DeltaW : kLevel32Delta)
+                                            : (wide ? kLevel32W : kLevel32);
     append(ph, kind, down, Level{na, ldp, d, np, PT, S, X, Sn, Sn ? npn : 0, gx}, gx * gy);
 }
 
@@ -4386,9 +4391,11 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kDense192: PARODE_RUN2((kg_chunks_dense<192, false>), (kg_chunks_dense<192, true>), Chunks, 768); break;
     case kDense224: PARODE_RUN2((kg_chunks_dense<224, false>), (kg_chunks_dense<224, true>), Chunks, 896); break;
     case kDense256: PARODE_RUN2((kg_chunks_dense<256, false>), (kg_chunks_dense<256, true>), Chunks, 1024); break;
-    case kLevel32: PARODE_RUN2((kg_level<float, false, false>), (kg_level<float, true, false>), Level, 256); break;
-    case kLevel32Delta: PARODE_RUN2((kg_level<float, false, true>), (kg_level<float, true, true>), Level, 256); break;
-    case kLevel64: PARODE_RUN2((kg_level<double, false, false>), (kg_level<double, true, false>), Level, 256); break;
+    case kLevel32: PARODE_RUN2((kg_level<float, false, false, true>), (kg_level<float, true, false, true>), Level, 256); break;
+    case kLevel32W: PARODE_RUN2((kg_level<float, false, false, false>), (kg_level<float, true, false, false>), Level, 256); break;
+    case kLevel32Delta: PARODE_RUN2((kg_level<float, false, true, true>), (kg_level<float, true, true, true>), Level, 256); break;
+    case kLevel32DeltaW: PARODE_RUN2((kg_level<float, false, true, false>), (kg_level<float, true, true, false>), Level, 256); break;
+    case kLevel64: PARODE_RUN2((kg_level<double, false, false, false>), (kg_level<double, true, false, false>), Level, 256); break;
     case kGather: PARODE_RUN(kg_gather, Gather, 256); break;
     case kTopMfma: PARODE_RUN(kg_top<true>, Top, kTopThreads); break;
     case kTopSimt: PARODE_RUN(kg_top<false>, Top, kTopThreads); break;
