@@ -498,7 +498,7 @@ def main():
     sec, ys, served = served_leg(args, torch, api, systems, dev)
     value = total / sec
     if args.only == "served":
-        print(json.dumps({"served_ms": sec * 1e3, "value": value}))
+        print(json.dumps({"served_ms": sec * 1e3, "value": value, **served}))
         return
     ys_host = {i: ys[i].cpu().numpy() for i in set(stratified(systems)) | set(range(8))}
     del ys

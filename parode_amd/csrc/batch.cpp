@@ -75,7 +75,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         maxL = std::max(maxL, s.L);
         upSteps = std::max(upSteps, s.lt - s.lc);
     }
-    const int kTables = kPow0 + maxL, kStart = kTables + 1, kLeaves = kStart + 1,
+    const int kTables = kPow0 + 4 * maxL, kStart = kTables + 1, kLeaves = kStart + 1,
               kChunk1 = kLeaves + 1, kUp0 = kChunk1 + 1, kTop = kUp0 + upSteps,
               kGather = kTop + 1, kDown0 = kGather + 1, kChunk2 = kDown0 + upSteps,
               kOut = kChunk2 + 1;
@@ -145,11 +145,30 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             if (!k::g::leaves_fit(n, m, 5, 2, 2, 2))
                 k::g::add_transpose(phases_[kPost], n, 5 * m, c.Bbar64_.get(), c.BbT64_.get());
         }
-        // the FP64 power table P_{l+1} = P_l^2 (transposed)
-        if (!k::g::add_powers_small(phases_[kPow0], na, L, PT))
-            for (int l = 0; l < L; ++l)
-                k::g::add_gemm(phases_[kPow0 + l], na, na, na, PT + l * na2, PT + l * na2,
-                               PT + (l + 1) * na2);
+        // the FP64 power table P_{l+1} = P_l^2 (transposed), up to P_{L-1}: the
+        // top of the tree reads levels below L, and one batch has no carry
+        if (!k::g::add_powers_small(phases_[kPow0], na, L, PT)) {
+            for (int l = 0; l + 1 < L; ++l) {
+                const double* Q = PT + l * na2;
+                double* Q2 = PT + (l + 1) * na2;
+                if (!abm) {
+                    k::g::add_gemm(phases_[kPow0 + 4 * l], na, na, na, Q, Q, Q2);
+                    continue;
+                }
+                // ABM: block row i of M^j is the last block row R_{j-3+i} of
+                // M^{j-3+i}, and R_{i+1} = R_i M.  Transposed, block column 0 of
+                // Q^2 (Q = P_l^T) is Q times Q's block column 0, and block column
+                // b is Q_0[:, 3n:4n] times the last n rows of block column b - 1,
+                // plus that column shifted down by n rows: 56 n^3 flop instead of
+                // the 128 n^3 of the full square.
+                k::g::add_gemm_ld(phases_[kPow0 + 4 * l], na, na, n, Q, na, Q, na, Q2, na,
+                                  nullptr, 0);
+                for (int b = 1; b < 4; ++b)
+                    k::g::add_gemm_ld(phases_[kPow0 + 4 * l + b], na, n, n, PT + 3 * n, na,
+                                      Q2 + 3L * n * na + (b - 1) * n, na, Q2 + b * n, na,
+                                      Q2 + (b - 1) * n, n);
+            }
+        }
         int n32 = 0, nd = 0;
         for (int l = 0; l < L; ++l) {
             if (!c.level_is_f64(l)) n32 = l + 1;
@@ -255,7 +274,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             // chunk pass; the rest after both
             const int pos = (int)pi;
             const int role = pos <= kPost ? 0 : pos <= kStart ? 1 : pos <= kChunk1 ? 2 : 3;
-            Ref r{&Lr, align(off), 0, stage_of(pos), role};
+            Ref r{&Lr, align(off), 0, stage_of(pos), role, pos};
             r.start = align(r.args + Lr.args.size());
             off = r.start + Lr.start.size() * sizeof(unsigned);
             refs_.push_back(r);
@@ -275,8 +294,30 @@ const char* BatchSolve::stage_name(int stage)
     return stage >= 0 && stage < kStages ? names[stage] : "";
 }
 
+void FanStreams::create()
+{
+    for (int k = 0; k < kCount; ++k) {
+        PARODE_HIP(hipStreamCreateWithFlags(&stream[k], hipStreamNonBlocking));
+        PARODE_HIP(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
+    }
+    PARODE_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+}
+
+void FanStreams::destroy()
+{
+    for (int k = 0; k < kCount; ++k) {
+        if (done[k]) (void)hipEventDestroy(done[k]);
+        if (stream[k]) (void)hipStreamDestroy(stream[k]);
+        done[k] = nullptr;
+        stream[k] = nullptr;
+    }
+    if (start) (void)hipEventDestroy(start);
+    start = nullptr;
+}
+
 void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
-                     const hipEvent_t* marks, const ScanCore::SideStream* side) const
+                     const hipEvent_t* marks, const ScanCore::SideStream* side,
+                     const FanStreams* fan) const
 {
     if (upload && !blob_.empty()) {
         std::memcpy(hostStage, blob_.data(), blob_.size());
@@ -286,30 +327,63 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
     auto launch = [&](hipStream_t st, const Ref& r) {
         k::g::run(st, *r.L, base + r.args, reinterpret_cast<const unsigned*>(base + r.start));
     };
+    auto wide = [](int stage) {
+        return stage == kStageInputs || stage == kStageLeaves || stage == kStageChunks1 ||
+               stage == kStageChunks2 || stage == kStageOutputs;
+    };
+    // the launches of one role in order; the launches of one wide phase spread
+    // over st and f's streams
+    auto queue = [&](hipStream_t st, int role, const FanStreams* f) {
+        for (size_t i = 0; i < refs_.size();) {
+            if (refs_[i].role != role) {
+                ++i;
+                continue;
+            }
+            size_t e = i + 1;
+            while (e < refs_.size() && refs_[e].phase == refs_[i].phase) ++e;
+            const int count = (int)(e - i);
+            const int used =
+                (f && wide(refs_[i].stage)) ? std::min(count - 1, FanStreams::kCount) : 0;
+            if (used > 0) {
+                PARODE_HIP(hipEventRecord(f->start, st));
+                for (int k = 0; k < used; ++k)
+                    PARODE_HIP(hipStreamWaitEvent(f->stream[k], f->start, 0));
+            }
+            for (size_t j = i; j < e; ++j) {
+                const int w = (int)((j - i) % (size_t)(used + 1));
+                launch(w == 0 ? st : f->stream[w - 1], refs_[j]);
+            }
+            for (int k = 0; k < used; ++k) {
+                PARODE_HIP(hipEventRecord(f->done[k], f->stream[k]));
+                PARODE_HIP(hipStreamWaitEvent(st, f->done[k], 0));
+            }
+            i = e;
+        }
+    };
     if (side && !marks) {
         if (!side->stream || !side->fork || !side->join)
             throw std::invalid_argument("BatchSolve::run: incomplete side stream");
-        for (const Ref& r : refs_)
-            if (r.role == 0) launch(s, r);
+        queue(s, 0, fan);
         PARODE_HIP(hipEventRecord(side->fork, s));
         PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
-        for (const Ref& r : refs_)
-            if (r.role == 1) launch(side->stream, r);
+        queue(side->stream, 1, nullptr);
         PARODE_HIP(hipEventRecord(side->join, side->stream));
-        for (const Ref& r : refs_)
-            if (r.role == 2) launch(s, r);
+        queue(s, 2, fan);
         PARODE_HIP(hipStreamWaitEvent(s, side->join, 0));
-        for (const Ref& r : refs_)
-            if (r.role == 3) launch(s, r);
+        queue(s, 3, fan);
+        return;
+    }
+    if (!marks) {
+        for (int role = 0; role < 4; ++role) queue(s, role, fan);
         return;
     }
     int stage = 0;
-    if (marks) PARODE_HIP(hipEventRecord(marks[0], s));
+    PARODE_HIP(hipEventRecord(marks[0], s));
     for (const Ref& r : refs_) {
-        while (marks && stage < r.stage) PARODE_HIP(hipEventRecord(marks[++stage], s));
+        while (stage < r.stage) PARODE_HIP(hipEventRecord(marks[++stage], s));
         launch(s, r);
     }
-    while (marks && stage < kStages) PARODE_HIP(hipEventRecord(marks[++stage], s));
+    while (stage < kStages) PARODE_HIP(hipEventRecord(marks[++stage], s));
 }
 
 }  // namespace parode
@@ -331,10 +405,12 @@ struct Group {
     bool uploaded = false;
     hipEvent_t marks[parode::BatchSolve::kStages + 1] = {};
     parode::ScanCore::SideStream side;
+    parode::FanStreams fan;
     ~Group()
     {
         batch.reset();
         arena.reset();
+        fan.destroy();
         if (side.fork) (void)hipEventDestroy(side.fork);
         if (side.join) (void)hipEventDestroy(side.join);
         if (side.stream) (void)hipStreamDestroy(side.stream);
@@ -419,8 +495,9 @@ int ParODE_GroupExecute(void* group, void* stream)
             PARODE_HIP(hipEventCreateWithFlags(&g->side.fork, hipEventDisableTiming));
             PARODE_HIP(hipEventCreateWithFlags(&g->side.join, hipEventDisableTiming));
         }
+        if (!g->fan.start) g->fan.create();
         g->batch->run(static_cast<hipStream_t>(stream), g->stage, g->desc, !g->uploaded, nullptr,
-                      &g->side);
+                      &g->side, &g->fan);
         g->uploaded = true;
         return ParODE_OK;
     } catch (std::exception& e) {

@@ -39,6 +39,18 @@ struct BatchSystem {
     float* Y = nullptr;             // points x p outputs
 };
 
+// Streams beside the caller's over which the launches of one wide phase (the
+// leaves, a chunk pass, the outputs: launches that never read each other's
+// results) are spread, so that one launch's last work-groups overlap the next
+// launch instead of leaving the GPU part idle.  Default priority.
+struct FanStreams {
+    static constexpr int kCount = 2;
+    hipStream_t stream[kCount] = {};
+    hipEvent_t start = nullptr, done[kCount] = {};
+    void create();     // throws std::runtime_error on a HIP failure
+    void destroy();
+};
+
 class BatchSolve {
 public:
     static bool eligible(int method, int n, int m, int p, bool zC, bool zD, long points);
@@ -61,10 +73,12 @@ public:
     // then runs on s, one after the other).  Else, with side, the FP64 power
     // tables, the level tables and the ABM start-up run on side->stream
     // (forked after the step map, joined before the first tree level) beside
-    // the leaves and the first chunk pass.
+    // the leaves and the first chunk pass; with fan, too, the launches of the
+    // inputs, leaves, chunk-pass and output phases are spread over s and the
+    // fan streams (each phase forked from s and joined back to it).
     void run(hipStream_t s, void* hostStage, void* devDesc, bool upload = true,
-             const hipEvent_t* marks = nullptr,
-             const ScanCore::SideStream* side = nullptr) const;
+             const hipEvent_t* marks = nullptr, const ScanCore::SideStream* side = nullptr,
+             const FanStreams* fan = nullptr) const;
     int launches() const { return (int)refs_.size(); }
 
     // Stages of a grouped solve, in launch order (for timing).
@@ -87,6 +101,7 @@ private:
         size_t args, start;
         int stage;
         int role;    // 0 map, 1 side stream, 2 beside the side stream, 3 after the join
+        int phase;
     };
     std::vector<Ref> refs_;
 };

@@ -261,6 +261,10 @@ void add_fill(Phase& ph, long count, double v, double* out);
 // out[r][c] = in[c][r] for c < cols (in: cols x rows), zero up to pp columns
 void add_transpose_pad(Phase& ph, int rows, int cols, int pp, const float* in, double* out);
 void add_gemm(Phase& ph, int r, int kk, int c, const double* X, const double* Y, double* Z);
+// Z = X Y with leading dimensions ldx, ldy, ldz, plus, when add is given,
+// add[(i - shift) ldz + j] on the rows i >= shift
+void add_gemm_ld(Phase& ph, int r, int kk, int c, const double* X, int ldx, const double* Y,
+                 int ldy, double* Z, int ldz, const double* add, int shift);
 void add_rk4_combine(Phase& ph, int n, int m, double h, const double* A, const double* A2,
                      const double* A3, const double* A4, const double* B, const double* AB,
                      const double* A2B, const double* A3B, double* MT, double* Bbar);

@@ -159,11 +159,26 @@ __global__ void k_gemm_f64_mfma(int r, int kk, int c, const double* __restrict__
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kGemmKsT = 32, kGemmKsKB = 32;
 
+// Leading dimensions of the FP64 GEMMs' operands (X r x kk, Y kk x c, Z r x c,
+// row-major), and an optional addend: Z[i][j] = (X Y)[i][j] + add[(i - shift)
+// ldz + j] for i >= shift (the ABM companion's shifted blocks).
+struct GemmLd {
+    int ldx, ldy, ldz;
+    const double* add;
+    int shift;
+};
+__host__ __device__ inline GemmLd gemm_ld(int kk, int c) { return GemmLd{kk, c, c, nullptr, 0}; }
+__device__ __forceinline__ double gemm_out(const GemmLd& ld, int i, int j, double v)
+{
+    return (ld.add && i >= ld.shift) ? v + ld.add[(long)(i - ld.shift) * ld.ldz + j] : v;
+}
+
 template <int KSPLIT>
 __device__ __forceinline__ void d_gemm_f64_ks(const VBlock vb, int r, int kk, int c,
                                                               const double* __restrict__ X,
                                                               const double* __restrict__ Y,
-                                                              double* __restrict__ Z)
+                                                              double* __restrict__ Z,
+                                                              const GemmLd ld)
 {
     constexpr int T = kGemmKsT, KB = kGemmKsKB, AP = KB + 1, BP = T + 1, NT = 256 * KSPLIT;
     constexpr int PA = T * KB / NT, PB = KB * T / NT;
@@ -182,12 +197,12 @@ __device__ __forceinline__ void d_gemm_f64_ks(const VBlock vb, int r, int kk, in
 #pragma unroll
         for (int q = 0; q < PA; ++q) {
             const int gi = i0 + ar, gk = k0 + ak + q;
-            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
+            ra[q] = (gi < r && gk < kk) ? X[(long)gi * ld.ldx + gk] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < PB; ++q) {
             const int gk2 = k0 + bk, gj = j0 + bj + q;
-            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
+            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * ld.ldy + gj] : 0.0;
         }
     };
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -222,7 +237,7 @@ __device__ __forceinline__ void d_gemm_f64_ks(const VBlock vb, int r, int kk, in
 #pragma unroll
         for (int g = 1; g < KSPLIT; ++g) y += part[(((g - 1) * 4 + wq) * 4 + v) * 64 + lane];
         const int i = i0 + wr * 16 + 4 * v + q4, j = j0 + wc * 16 + l16;
-        if (i < r && j < c) Z[(long)i * c + j] = y;
+        if (i < r && j < c) Z[(long)i * ld.ldz + j] = gemm_out(ld, i, j, y);
     }
 }
 
@@ -232,7 +247,8 @@ __global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int
                                                               const double* __restrict__ Y,
                                                               double* __restrict__ Z)
 {
-    d_gemm_f64_ks<KSPLIT>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z);
+    d_gemm_f64_ks<KSPLIT>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z,
+                          gemm_ld(kk, c));
 }
 
 // FP64 GEMM on 64 x 64 tiles of Z for the squarings of the power tables: four
@@ -246,7 +262,7 @@ constexpr int kG64 = 64, kG64KB = 16;
 __device__ __forceinline__ void d_gemm_f64_t64(const VBlock vb, int r, int kk, int c,
                                                const double* __restrict__ X,
                                                const double* __restrict__ Y,
-                                               double* __restrict__ Z)
+                                               double* __restrict__ Z, const GemmLd ld)
 {
     __shared__ double As[2][kG64][kG64KB + 1];
     __shared__ double Bs[2][kG64KB][kG64 + 1];
@@ -260,9 +276,9 @@ __device__ __forceinline__ void d_gemm_f64_t64(const VBlock vb, int r, int kk, i
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int gi = i0 + ar, gk = k0 + ak + q;
-            ra[q] = (gi < r && gk < kk) ? X[(long)gi * kk + gk] : 0.0;
+            ra[q] = (gi < r && gk < kk) ? X[(long)gi * ld.ldx + gk] : 0.0;
             const int gk2 = k0 + bk, gj = j0 + bj + q;
-            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * c + gj] : 0.0;
+            rb[q] = (gk2 < kk && gj < c) ? Y[(long)gk2 * ld.ldy + gj] : 0.0;
         }
     };
     auto store = [&](int buf) {
@@ -312,7 +328,7 @@ __device__ __forceinline__ void d_gemm_f64_t64(const VBlock vb, int r, int kk, i
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wr * 32 + ti * 16 + 4 * v + q4;
                 const int j = j0 + wc * 32 + tj * 16 + l16;
-                if (i < r && j < c) Z[(long)i * c + j] = acc[ti][tj][v];
+                if (i < r && j < c) Z[(long)i * ld.ldz + j] = gemm_out(ld, i, j, acc[ti][tj][v]);
             }
 }
 
@@ -351,7 +367,7 @@ void gemm_f64(hipStream_t s, int r, int kk, int c, const double* X, const double
 __device__ __forceinline__ void d_gemm_f64_mfma(const VBlock vb, int r, int kk, int c,
                                                        const double* __restrict__ X,
                                                        const double* __restrict__ Y,
-                                                       double* __restrict__ Z)
+                                                       double* __restrict__ Z, const GemmLd ld)
 {
     __shared__ double part[3][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -369,8 +385,8 @@ __device__ __forceinline__ void d_gemm_f64_mfma(const VBlock vb, int r, int kk, 
         for (int u = 0; u < U; ++u) {
             const int kx = (k + u) * 4 + q4;
             const bool ok = k + u < ke && kx < kk;
-            a[u] = (ok && ia < r) ? X[(long)ia * kk + kx] : 0.0;
-            b[u] = (ok && jb < c) ? Y[(long)kx * c + jb] : 0.0;
+            a[u] = (ok && ia < r) ? X[(long)ia * ld.ldx + kx] : 0.0;
+            b[u] = (ok && jb < c) ? Y[(long)kx * ld.ldy + jb] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
@@ -386,7 +402,7 @@ __device__ __forceinline__ void d_gemm_f64_mfma(const VBlock vb, int r, int kk, 
         for (int v = 0; v < 4; ++v) {
             const double sum = acc[v] + part[0][v][lane] + part[1][v][lane] + part[2][v][lane];
             const int i = i0 + 4 * v + q4, j = j0 + l16;
-            if (i < r && j < c) Z[(long)i * c + j] = sum;
+            if (i < r && j < c) Z[(long)i * ld.ldz + j] = gemm_out(ld, i, j, sum);
         }
     }
 }
@@ -396,7 +412,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64_mfma(int r, int kk, int c,
                                                        const double* __restrict__ Y,
                                                        double* __restrict__ Z)
 {
-    d_gemm_f64_mfma(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z);
+    d_gemm_f64_mfma(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, r, kk, c, X, Y, Z,
+                    gemm_ld(kk, c));
 }
 
 __device__ __forceinline__ void d_cast_f32_f64(const VBlock vb, long count, const float* __restrict__ in, double* __restrict__ out)
@@ -3670,6 +3687,7 @@ struct Gemm {
     int r, kk, c;
     const double *X, *Y;
     double* Z;
+    GemmLd ld;
     long gx;
 };
 struct Rk4Comb {
@@ -3825,21 +3843,21 @@ __global__ __launch_bounds__(256) void kg_gemm16(const Gemm* __restrict__ args,
                                                  const unsigned* __restrict__ st, int ns)
 {
     PARODE_GROUP(Gemm);
-    d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+    d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
 }
 
 __global__ __launch_bounds__(256) void kg_gemm_t64(const Gemm* __restrict__ args,
                                                    const unsigned* __restrict__ st, int ns)
 {
     PARODE_GROUP(Gemm);
-    d_gemm_f64_t64(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+    d_gemm_f64_t64(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
 }
 
 __global__ __launch_bounds__(512) void kg_gemm_ks2(const Gemm* __restrict__ args,
                                                    const unsigned* __restrict__ st, int ns)
 {
     PARODE_GROUP(Gemm);
-    d_gemm_f64_ks<2>(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z);
+    d_gemm_f64_ks<2>(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
 }
 
 __global__ __launch_bounds__(256) void kg_rk4_combine(const Rk4Comb* __restrict__ args,
@@ -4082,18 +4100,22 @@ void add_transpose_pad(Phase& ph, int rows, int cols, int pp, const float* in, d
 
 void add_gemm(Phase& ph, int r, int kk, int c, const double* X, const double* Y, double* Z)
 {
+    add_gemm_ld(ph, r, kk, c, X, kk, Y, c, Z, c, nullptr, 0);
+}
+
+void add_gemm_ld(Phase& ph, int r, int kk, int c, const double* X, int ldx, const double* Y,
+                 int ldy, double* Z, int ldz, const double* add, int shift)
+{
     if (r <= 0 || c <= 0) return;
-    if (r >= 256 && c >= 256 && kk >= 64) {   // the squarings of wide power tables
+    const GemmLd ld{ldx, ldy, ldz, add, shift};
+    // 64 x 64 tiles once the padding of the edge tiles costs less than the
+    // 16 x 16 kernel's lower rate (about half)
+    if (r >= 96 && c >= 96 && kk >= 64) {
         const long gx = (c + kG64 - 1) / kG64, gy = (r + kG64 - 1) / kG64;
-        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy);
-    } else if (r >= 320 && c >= 320 && kk >= 64) {
-        const long gx = (c + kGemmKsT - 1) / kGemmKsT, gy = (r + kGemmKsT - 1) / kGemmKsT;
-        const size_t stage = (size_t)(kGemmKsT * (kGemmKsKB + 1) + kGemmKsKB * (kGemmKsT + 1));
-        append(ph, kGemmKs2, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy,
-               std::max<size_t>(stage, 1 * 4 * 4 * 64) * sizeof(double));
+        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy);
     } else {
         const long gx = (c + 15) / 16, gy = (r + 15) / 16;
-        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, gx}, gx * gy);
+        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy);
     }
 }
 

@@ -211,7 +211,7 @@ void ScanCore::build_map(hipStream_t s, const float* dA, const float* dB, double
     }
 }
 
-void ScanCore::build_powers(hipStream_t s)
+void ScanCore::build_powers(hipStream_t s, bool withLast)
 {
     const size_t na2 = (size_t)na_ * na_;
     // (P^T)^2 = (P^2)^T: squaring the transposed table keeps it transposed
@@ -219,7 +219,7 @@ void ScanCore::build_powers(hipStream_t s)
     if (mm && k::powers_small_supported(na_)) {
         k::powers_small(s, na_, L_, PT64_.get());
     } else {
-        for (int l = 0; l < L_; ++l)
+        for (int l = 0; l < (withLast ? L_ : L_ - 1); ++l)
             k::gemm_f64(s, na_, na_, na_, PT64_.get() + l * na2, PT64_.get() + l * na2,
                         PT64_.get() + (l + 1) * na2, mm);
     }
@@ -775,7 +775,7 @@ void ScanCore::solve(hipStream_t s, long points, double h, const float* dA, cons
         leaves(s, dU, uDouble, 2, 0, 2, nScan, nPad);
     tick(2);
     hipStream_t ps = side ? side->stream : s;
-    build_powers(ps);
+    build_powers(ps, false);   // one batch: no carry, so no P_L
     // the tree levels wait for the tables only; the root for the seed, the
     // outputs for their tables
     const bool late = side && side->late;

@@ -66,7 +66,8 @@ public:
     };
     void build_map(hipStream_t s, const float* dA, const float* dB, double h,
                    const SideStream* side = nullptr);
-    void build_powers(hipStream_t s);
+    // withLast: also P_L = M^(2^L), which only carry() between batches reads
+    void build_powers(hipStream_t s, bool withLast = true);
     void leaves(hipStream_t s, const void* dU, bool uDouble, long sL, long oL, long rs, long nReal,
                 long nPad);
     // The scan of one batch runs as: up-sweep launches for the levels below
