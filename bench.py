@@ -110,6 +110,24 @@ def level_flops(na, n_scan, n_leaf=None, chunk_log=0, factored=False):
     return flops, pairs
 
 
+def system_bound_seconds(method, n, m, p, points):
+    """Roofline time of one system's mixed-precision solve as built: FP32
+    matrix-core flops (chunk passes and tree levels), FP64 matrix-core flops
+    (the power squarings up to P_{L-1}: 2 n_a^3 each, 56 n^3 for the ABM
+    companion; the leaves, 2 n k m per step with k = 3 for RK4 and 2 for the
+    factored ABM; the outputs, 2 p (n + m) per point), the two kinds sharing
+    the matrix cores, against the compulsory HBM bytes (FP64 samples on the h/2
+    grid in, FP32 outputs out)."""
+    n_scan, _, L, na, lc, _ = scan_shape(method, n, points)
+    f32, _ = level_flops(na, n_scan, n_leaf=n if method else None, chunk_log=lc,
+                         factored=bool(method))
+    sq = 56.0 * n ** 3 if method else 2.0 * na ** 3
+    f64 = max(L - 1, 0) * sq + 2.0 * n * (2 if method else 3) * m * n_scan \
+        + 2.0 * p * (n + m) * points
+    hbm = (2 * points + 1) * m * 8.0 + points * p * 4.0
+    return max(f32 / PEAK["fp32"] + f64 / PEAK["fp64"], hbm / PEAK["hbm"])
+
+
 # ------------------------------------------------------------------- helpers
 def _sync_time(torch, fn, reps):
     torch.cuda.synchronize()
@@ -512,6 +530,7 @@ def main():
     cflops = sum(chunk_flops(s.method, s.n, s.steps) for s in systems)
     chunk_ms = grp["stage_ms"]["chunks_pass1"] + grp["stage_ms"]["chunks_pass2"]
     achieved = cflops / (chunk_ms * 1e-3)
+    bound = total / sum(system_bound_seconds(s.method, s.n, s.m, s.p, s.steps) for s in systems)
 
     line = {
         "metric": METRIC,
@@ -537,6 +556,12 @@ def main():
         "plain": {"ms": plain_ms, "state_steps_per_s": total / (plain_ms * 1e-3),
                   "precision": "plain fp32", "launch": "one system per solve, one stream"},
         "group": grp,
+        "batch_roofline": {
+            "bound_state_steps_per_s": bound,
+            "frac": value / bound,
+            "model": "sum N_i / sum max(F32_i / 157.3 TF + F64_i / 78.6 TF, B_i / 8 TB/s) with "
+                     "the as-built flops (chunk passes, tree levels, FP64 squarings, leaves, "
+                     "outputs) and the compulsory bytes (FP64 samples in, FP32 outputs out)"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,
                      "kernel": "grouped chunk recurrences (kg_chunks_fac / kg_chunks_two / "

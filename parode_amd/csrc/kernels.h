@@ -233,7 +233,8 @@ void to_f64(hipStream_t s, long count, const T* in, double* out);
 namespace g {
 enum Kind {
     kPrep, kGemm16, kGemmKs2, kGemmT64, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
-    kSample, kStartup, kStartOut, kLeavesPlain, kLeavesFac, kLeavesSimt,
+    kSample, kStartup, kStartOut, kLeavesPlain, kLeavesFac, kLeavesSimt, kOutputsGemm1,
+    kOutputsGemm2, kOutputsGemm4,
     kChunksFac16, kChunksFac32, kChunksFac48, kChunksFac64, kChunksFac96, kChunksFac128,
     kChunksTwo160, kChunksTwo192, kChunksTwo224, kChunksTwo256,
     kDense16, kDense32, kDense48, kDense64, kDense96, kDense128, kDense160, kDense192, kDense224,

@@ -1764,6 +1764,30 @@ void level_mfma(hipStream_t s, bool down, bool delta, int na, long d, long np, c
 //   Element e's leaf v_e is read from X[e] before x_e is written there.
 constexpr int kChunkCols = 16;   // chunks per work-group
 
+// LDS rows of the chunk kernels (one per chunk): lane group g = lane / 16 reads
+// the KS entries [g KS, (g + 1) KS) of a row as 16-byte fragments, one row per
+// lane of the group.  The LDS serves such a read 32 lanes at a time over 128
+// banks (measured: the rows' old layout, groups at g KS with a pitch of 4 mod
+// 128 words, had two bank conflicts per access unless KS = 64).  From KS = 12
+// on, group g's entries start at chunk_base(g): 0, 64, 32, 96 (KS <= 32) or
+// g 64 (KS <= 64), with a pitch of 4 mod 128 words, so that the 16 lanes of a
+// group take distinct bank quads and the two groups served together take
+// disjoint halves of the banks.  Narrower rows keep entries in order.
+__host__ __device__ constexpr bool chunk_spread(int ks) { return ks >= 12 && ks <= 64; }
+__host__ __device__ constexpr int chunk_pitch(int ks, int entries)
+{
+    return chunk_spread(ks) ? (ks <= 32 ? 132 : 260) : entries + 4;
+}
+__host__ __device__ constexpr int chunk_base(int ks, int g)
+{
+    return chunk_spread(ks) ? (ks <= 32 ? (g & 1) * 64 + (g >> 1) * 32 : g * 64) : g * ks;
+}
+// LDS offset of entry e of a row
+__host__ __device__ constexpr int chunk_pos(int ks, int e)
+{
+    return chunk_base(ks, e / ks) + e % ks;
+}
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <int NP, bool PASS2>
@@ -1916,7 +1940,7 @@ __device__ __forceinline__ void d_abm_chunks_fac(const VBlock vb, int n, int lc,
 #pragma clang fp contract(off)
     constexpr int CH = kChunkCols;
     constexpr int KS = NP / 2;            // k-steps of four over the 2 NP padded [w1 w2]
-    constexpr int PITCH = 2 * NP + 4;     // 4 mod 64 words for NP = 64; 16-byte aligned rows
+    constexpr int PITCH = chunk_pitch(KS, 2 * NP);   // 16-byte aligned rows
     extern __shared__ float wbuf[];       // [2][CH][PITCH]
     const int na = 4 * n, Lc = 1 << lc;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1966,8 +1990,8 @@ __device__ __forceinline__ void d_abm_chunks_fac(const VBlock vb, int n, int lc,
                 w2 = w2 + f.b[b] * hs[v][b];
             }
             if (r < NP) {
-                wbuf[(buf * CH + c) * PITCH + r] = rowOk ? w1 : 0.0f;
-                wbuf[(buf * CH + c) * PITCH + NP + r] = rowOk ? w2 : 0.0f;
+                wbuf[(buf * CH + c) * PITCH + chunk_pos(KS, r)] = rowOk ? w1 : 0.0f;
+                wbuf[(buf * CH + c) * PITCH + chunk_pos(KS, NP + r)] = rowOk ? w2 : 0.0f;
             }
         }
     };
@@ -1992,7 +2016,7 @@ __device__ __forceinline__ void d_abm_chunks_fac(const VBlock vb, int n, int lc,
             vn[v] = (rowOk && more) ? X[((c0 + cl[v]) * Lc + jn) * na + 3 * n + r] : 0.0f;
         floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
         const floatx4* ab =
-            reinterpret_cast<const floatx4*>(wbuf + (cur * CH + l16) * PITCH + g * KS);
+            reinterpret_cast<const floatx4*>(wbuf + (cur * CH + l16) * PITCH + chunk_base(KS, g));
         floatx4 a[KS / 4];
 #pragma unroll
         for (int q = 0; q < KS / 4; ++q) a[q] = ab[q];
@@ -2065,7 +2089,7 @@ __device__ __forceinline__ void d_abm_chunks_two(const VBlock vb, int n, int lc,
 #pragma clang fp contract(off)
     constexpr int CH = kChunkCols;
     constexpr int KS = NP / 4;            // padded states per lane group
-    constexpr int PITCH = NP + 4;         // 16-byte aligned rows
+    constexpr int PITCH = chunk_pitch(KS, NP);   // 16-byte aligned rows
     static_assert(NP % 32 == 0 && KS % 4 == 0, "NP: a multiple of 32");
     extern __shared__ float sbuf[];       // w2: [CH][PITCH], then u: [CH][PITCH]
     float* const w2buf = sbuf;
@@ -2124,14 +2148,15 @@ __device__ __forceinline__ void d_abm_chunks_two(const VBlock vb, int n, int lc,
                     b2 = b2 + f.b[b] * hs[t][v][b];
                 }
                 w1[t][v] = a;
-                w2buf[(4 * g + v) * PITCH + rr[t]] = ok[t] ? b2 : 0.0f;
+                w2buf[(4 * g + v) * PITCH + chunk_pos(KS, rr[t])] = ok[t] ? b2 : 0.0f;
             }
     };
     // 16 chunks x NP padded states of `buf` times this wave's rows of A
     auto product = [&](const float* buf, floatx4 (&acc)[2]) {
         acc[0] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
         acc[1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        const floatx4* ab = reinterpret_cast<const floatx4*>(buf + l16 * PITCH + g * KS);
+        const floatx4* ab =
+            reinterpret_cast<const floatx4*>(buf + l16 * PITCH + chunk_base(KS, g));
 #pragma unroll
         for (int q = 0; q < KS / 4; ++q) {
             const floatx4 a = ab[q];
@@ -2169,7 +2194,8 @@ __device__ __forceinline__ void d_abm_chunks_two(const VBlock vb, int n, int lc,
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                ubuf[(4 * g + v) * PITCH + rr[t]] = ok[t] ? w1[t][v] + acc[t][v] : 0.0f;
+                ubuf[(4 * g + v) * PITCH + chunk_pos(KS, rr[t])] =
+                    ok[t] ? w1[t][v] + acc[t][v] : 0.0f;
         __syncthreads();
         product(ubuf, acc);
 #pragma unroll
@@ -2237,7 +2263,7 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
     const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
     auto launch = [&](auto np) {
         constexpr int NP = decltype(np)::value;
-        const size_t lds = (size_t)2 * kChunkCols * (2 * NP + 4) * sizeof(float);
+        const size_t lds = (size_t)2 * kChunkCols * chunk_pitch(NP / 2, 2 * NP) * sizeof(float);
         if (pass2)
             hipLaunchKernelGGL((k_abm_chunks_fac<NP, true>), grid, dim3(NP * 4), lds, s, n, lc,
                                nReal, A64, A264, f, X);
@@ -2247,7 +2273,7 @@ void abm_chunks_factored(hipStream_t s, bool pass2, int n, int lc, long nReal, l
     };
     auto launch_two = [&](auto np) {
         constexpr int NP = decltype(np)::value;
-        const size_t lds = (size_t)2 * kChunkCols * (NP + 4) * sizeof(float);
+        const size_t lds = (size_t)2 * kChunkCols * chunk_pitch(NP / 4, NP) * sizeof(float);
         if (pass2)
             hipLaunchKernelGGL((k_abm_chunks_two<NP, true>), grid, dim3(NP * 2), lds, s, n, lc,
                                nReal, A64, f, X);
@@ -2299,7 +2325,7 @@ __device__ __forceinline__ void d_chunks_dense(const VBlock vb, int na, int lc, 
 #pragma clang fp contract(off)
     constexpr int CH = kChunkCols;
     constexpr int KS = NP / 4;            // k-steps of four states
-    constexpr int PITCH = NP + 4;         // 16-byte aligned rows, 16 chunks in distinct banks
+    constexpr int PITCH = chunk_pitch(KS, NP);   // 16-byte aligned rows
     constexpr int NT = NP * 4;
     extern __shared__ float xsd[];        // [CH][PITCH]
     const int Lc = 1 << lc;
@@ -2339,7 +2365,7 @@ __device__ __forceinline__ void d_chunks_dense(const VBlock vb, int na, int lc, 
     __syncthreads();
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-        if (rowOk) xsd[(4 * g + v) * PITCH + r] = xh[v];
+        if (rowOk) xsd[(4 * g + v) * PITCH + chunk_pos(KS, r)] = xh[v];
     // pass 2: the state of the chunk's first element goes out (over its leaf)
     if (PASS2 && rowOk)
 #pragma unroll
@@ -2358,7 +2384,8 @@ __device__ __forceinline__ void d_chunks_dense(const VBlock vb, int na, int lc, 
         for (int v = 0; v < 4; ++v)
             vn[v] = (rowOk && more) ? X[pos((c0 + cl[v]) * Lc + jn) * na + r] : 0.0f;
         floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-        const floatx4* ab = reinterpret_cast<const floatx4*>(xsd + l16 * PITCH + g * KS);
+        const floatx4* ab =
+            reinterpret_cast<const floatx4*>(xsd + l16 * PITCH + chunk_base(KS, g));
 #pragma unroll
         for (int q = 0; q < KS / 4; ++q) {
             const floatx4 a = ab[q];
@@ -2375,7 +2402,7 @@ __device__ __forceinline__ void d_chunks_dense(const VBlock vb, int na, int lc, 
             const float sum = xh[v] + y;
             xc[v] = (sum - xh[v]) - y;
             xh[v] = sum;
-            if (rowOk) xsd[c * PITCH + r] = sum;
+            if (rowOk) xsd[c * PITCH + chunk_pos(KS, r)] = sum;
             if (PASS2 && rowOk) {
                 const long e = (c0 + c) * Lc + j + 1;
                 if (c < cReal && e < nReal) X[pos(e) * na + r] = sum - xc[v];
@@ -2412,7 +2439,7 @@ void chunks_dense(hipStream_t s, bool pass2, int na, int lc, long S, long nReal,
     const dim3 grid((unsigned)((nChunks + kChunkCols - 1) / kChunkCols));
     auto launch = [&](auto np) {
         constexpr int NP = decltype(np)::value;
-        const size_t lds = (size_t)kChunkCols * (NP + 4) * sizeof(float);
+        const size_t lds = (size_t)kChunkCols * chunk_pitch(NP / 4, NP) * sizeof(float);
         if (pass2)
             hipLaunchKernelGGL((k_chunks_dense<NP, true>), grid, dim3(NP * 4), lds, s, na, lc, S,
                                nReal, P64, X);
@@ -3365,6 +3392,111 @@ void pad_cast(hipStream_t s, int rows, int cols, int pp, const float* in, double
     check_launch("pad_cast");
 }
 
+// Outputs as tiles of 64 elements x 16 JT outputs of the product
+// Y[e][j] = sum_k [x_e | u_e][k] [C^T ; D^T][k][j] on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64; the FP32 states convert exactly): four waves, each
+// 16 elements x JT output tiles (one A fragment feeds JT products), K = the n
+// states (with C) then the m inputs (with D) in slices of 16 staged in LDS
+// and double buffered.  Elements past N read zero and are not stored;
+// output columns past pp (the padding of C^T and D^T) read zero.  Grid: x =
+// output tile, y = element tile.
+constexpr int kOG = 64, kOGKB = 16;
+
+template <typename TX, typename TU, int JT>
+__device__ __forceinline__ void d_outputs_gemm(const VBlock vb, int n, int m, int p, int pp,
+                                               int na, int oX, bool useC, bool useD, long N,
+                                               const double* __restrict__ CT,
+                                               const double* __restrict__ DT,
+                                               const TX* __restrict__ X,
+                                               const TU* __restrict__ U, long sO, long oO,
+                                               float* __restrict__ Y)
+{
+    constexpr int OW = 16 * JT;                      // outputs per tile
+    constexpr int BQ = OW / 16;                      // B values per thread and slice
+    __shared__ double As[2][kOG][kOGKB + 1];
+    __shared__ double Bs[2][kOGKB][OW + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, q4 = lane >> 4;
+    const long e0 = vb.y * kOG;
+    const int j0 = (int)vb.x * OW;
+    const int kC = useC ? n : 0, K = kC + (useD ? m : 0);
+    const int ar = tid >> 2, ak = (tid & 3) * 4;     // A slice: element, first of four k
+    const int bk = tid >> 4, bj = (tid & 15) * BQ;   // B slice: k, first of BQ outputs
+    const long ea = e0 + ar;
+    const bool realA = ea < N;
+    const TX* xe = X + ea * na + (long)oX * n;
+    const TU* ue = U + (ea * sO + oO) * m;
+    double ra[4], rb[BQ];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + ak + q;
+            ra[q] = (realA && k < K) ? (k < kC ? (double)xe[k] : (double)ue[k - kC]) : 0.0;
+        }
+        const int kb = k0 + bk;
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int j = j0 + bj + q;
+            rb[q] = (kb < K && j < pp) ? (kb < kC ? CT[(long)kb * pp + j]
+                                                  : DT[(long)(kb - kC) * pp + j])
+                                       : 0.0;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) As[buf][ar][ak + q] = ra[q];
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) Bs[buf][bk][bj + q] = rb[q];
+    };
+    doublex4 acc[JT];
+#pragma unroll
+    for (int t = 0; t < JT; ++t) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += kOGKB) {
+        const bool more = k0 + kOGKB < K;
+        if (more) load(k0 + kOGKB);
+#pragma unroll
+        for (int ks = 0; ks < kOGKB; ks += 4) {
+            const double a = As[buf][wave * 16 + l16][ks + q4];
+#pragma unroll
+            for (int t = 0; t < JT; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[buf][ks + q4][t * 16 + l16],
+                                                             acc[t], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // D[i][j] (element i, output j) is in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int t = 0; t < JT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long e = e0 + wave * 16 + 4 * v + q4;
+            const int j = j0 + t * 16 + l16;
+            if (e < N && j < p) Y[e * p + j] = (float)acc[t][v];
+        }
+}
+
+template <typename TX, typename TU, int JT>
+__global__ __launch_bounds__(256) void k_outputs_gemm(int n, int m, int p, int pp, int na, int oX,
+                                                      bool useC, bool useD, long N,
+                                                      const double* __restrict__ CT,
+                                                      const double* __restrict__ DT,
+                                                      const TX* __restrict__ X,
+                                                      const TU* __restrict__ U, long sO, long oO,
+                                                      float* __restrict__ Y)
+{
+    d_outputs_gemm<TX, TU, JT>(VBlock{(long)blockIdx.x, (long)blockIdx.y, (long)gridDim.x}, n, m,
+                               p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
+}
+
+// output tiles of 16, 32 or 64 outputs
+static int outputs_gemm_width(int p) { return p <= 16 ? 1 : p <= 32 ? 2 : 4; }
+
 template <typename TX, typename TU>
 bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, bool zC, bool zD,
                    long N, const double* CT, const double* DT, const TX* X, const TU* U, long sO,
@@ -3374,6 +3506,24 @@ bool outputs_lanes(hipStream_t s, int n, int m, int p, int pp, int na, int oX, b
     if (N <= 0 || p <= 0) return N <= 0;
     if (zC && zD) return false;   // the outputs are the states: k_outputs copies them
     if (pp % 8 != 0 || pp < p) throw std::invalid_argument("outputs_lanes: bad padding");
+    if (useC && !(sizeof(TX) == 4 && n <= kOutLanes)) {
+        // wide (or FP64) states: 64-element tiles on the FP64 matrix cores
+        const int jt = outputs_gemm_width(p);
+        const dim3 g2((unsigned)((p + 16 * jt - 1) / (16 * jt)), (unsigned)((N + kOG - 1) / kOG));
+        auto go = [&](auto w) {
+            constexpr int JT = decltype(w)::value;
+            hipLaunchKernelGGL((k_outputs_gemm<TX, TU, JT>), g2, dim3(256), 0, s, n, m, p, pp, na,
+                               oX, useC, useD, N, CT, DT, X, U, sO, oO, Y);
+        };
+        if (jt == 1)
+            go(std::integral_constant<int, 1>());
+        else if (jt == 2)
+            go(std::integral_constant<int, 2>());
+        else
+            go(std::integral_constant<int, 4>());
+        check_launch("outputs_gemm");
+        return true;
+    }
     // persistent work-groups (several blocks each)
     const long blocks = std::min<long>((N + kOutLanes - 1) / kOutLanes, 4096);
     const dim3 grid((unsigned)std::max<long>(blocks, 1));
@@ -3941,6 +4091,15 @@ __global__ __launch_bounds__(256, 3) void kg_leaves(const Leaves* __restrict__ a
                                       a.U, a.sL, a.oL, a.rs, a.X);
 }
 
+template <int JT>
+__global__ __launch_bounds__(256) void kg_outputs_gemm(const Outputs* __restrict__ args,
+                                                       const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Outputs);
+    d_outputs_gemm<float, double, JT>(vb, a.n, a.m, a.p, a.pp, a.na, a.oX, a.useC, a.useD, a.N,
+                                      a.CT, a.DT, a.X, a.U, a.sO, a.oO, a.Y);
+}
+
 __global__ __launch_bounds__(256) void kg_leaves_simt(const Leaves* __restrict__ args,
                                                       const unsigned* __restrict__ st, int ns)
 {
@@ -4272,8 +4431,9 @@ bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad,
     int t = 0;
     while (kNp[t] < n) ++t;
     const int np = kNp[t];
-    const size_t lds = np <= 128 ? (size_t)2 * kChunkCols * (2 * np + 4) * sizeof(float)
-                                 : (size_t)2 * kChunkCols * (np + 4) * sizeof(float);
+    const size_t lds = np <= 128
+                           ? (size_t)2 * kChunkCols * chunk_pitch(np / 2, 2 * np) * sizeof(float)
+                           : (size_t)2 * kChunkCols * chunk_pitch(np / 4, np) * sizeof(float);
     append(ph, kKind[t], pass2, a, b, lds);
     return true;
 }
@@ -4290,7 +4450,7 @@ bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nP
     int t = 0;
     while (kNp[t] < na) ++t;
     append(ph, kKind[t], pass2, Chunks{na, lc, nReal, P64, nullptr, AbmFactors{}, X, b}, b,
-           (size_t)kChunkCols * (kNp[t] + 4) * sizeof(float));
+           (size_t)kChunkCols * chunk_pitch(kNp[t] / 4, kNp[t]) * sizeof(float));
     return true;
 }
 
@@ -4345,7 +4505,13 @@ bool add_outputs(Phase& ph, int n, int m, int p, int pp, int na, int oX, bool us
     if (p <= 0 || pp % 8 != 0 || pp < p) return false;
     const long b = std::max<long>(std::min<long>((N + kOutLanes - 1) / kOutLanes, 1024), 1);
     const Outputs a{n, m, p, pp, na, oX, useC, useD, N, CT, DT, X, U, sO, oO, Y, b};
-    if (n >= 32) {   // wide states: the FP64 matrix cores
+    if (useC && n >= 32) {   // wide states: 64-element tiles on the FP64 matrix cores
+        const int jt = outputs_gemm_width(p);
+        Outputs ag = a;
+        ag.gx = (p + 16 * jt - 1) / (16 * jt);
+        append(ph, jt == 1 ? kOutputsGemm1 : jt == 2 ? kOutputsGemm2 : kOutputsGemm4, false, ag,
+               ag.gx * ((N + kOG - 1) / kOG));
+    } else if (n >= 32) {   // wide states without C: the FP64 matrix cores
         const long bm = std::max<long>(std::min<long>((N + 63) / 64, 512), 1);
         Outputs am = a;
         am.gx = bm;
@@ -4393,6 +4559,9 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kLeavesPlain: PARODE_RUN(kg_leaves<false>, Leaves, 256); break;
     case kLeavesFac: PARODE_RUN(kg_leaves<true>, Leaves, 256); break;
     case kLeavesSimt: PARODE_RUN(kg_leaves_simt, Leaves, 256); break;
+    case kOutputsGemm1: PARODE_RUN(kg_outputs_gemm<1>, Outputs, 256); break;
+    case kOutputsGemm2: PARODE_RUN(kg_outputs_gemm<2>, Outputs, 256); break;
+    case kOutputsGemm4: PARODE_RUN(kg_outputs_gemm<4>, Outputs, 256); break;
     case kChunksFac16: PARODE_RUN2((kg_chunks_fac<16, false>), (kg_chunks_fac<16, true>), Chunks, 64); break;
     case kChunksFac32: PARODE_RUN2((kg_chunks_fac<32, false>), (kg_chunks_fac<32, true>), Chunks, 128); break;
     case kChunksFac48: PARODE_RUN2((kg_chunks_fac<48, false>), (kg_chunks_fac<48, true>), Chunks, 192); break;

@@ -276,11 +276,16 @@ def test_precision_modes_against_fp64(solver, mode, bound):
 @pytest.mark.parametrize("solver,n,m,steps,f32", [(1, 17, 1, 1000, False), (1, 40, 3, 3001, True),
                                                   (0, 5, 7, 999, False), (0, 70, 5, 2049, True),
                                                   (1, 64, 8, 4097, False),
-                                                  (1, 100, 4, 5000, False)])
+                                                  (1, 100, 4, 5000, False),
+                                                  (0, 128, 1, 3000, False),
+                                                  (1, 48, 33, 2500, True),
+                                                  (0, 200, 30, 1500, False)])
 def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
     """The FP64 leaves run on the matrix cores in 16 x 16 tiles: ragged row
     tiles (n not a multiple of 16), input counts that do not fill a 4-column
-    k-step, ragged horizons, FP32 and FP64 samples, against the FP64 oracle."""
+    k-step, one input, staged samples too large for the factored kernel (the
+    vector-ALU fallback), ragged horizons, FP32 and FP64 samples, against the
+    FP64 oracle."""
     s = synth.make_system(solver, n, m, 2, -6.0, -0.05, 1e-3, steps, seed=n + m)
     U = synth.samples_f64(s)
     if f32:
@@ -301,19 +306,23 @@ def test_fp64_leaves_shapes(solver, n, m, steps, f32, mode, bound):
 
 
 @pytest.mark.parametrize("mode,bound", [(api.FP64, 1e-7), (api.MIXED, 1e-6)])
-@pytest.mark.parametrize("solver,n,p", [(0, 200, 64), (1, 130, 40), (0, 90, 24), (1, 70, 17),
-                                        (0, 300, 5)])
-def test_wide_state_outputs_against_fp64(solver, n, p, mode, bound):
-    """Outputs of wide (n > 64) or FP64 states go through LDS in chunks of 32
-    states with the outputs in groups of 32, 24, 16 or 8 (whichever divides
-    the padded count): each group size, ragged chunks and ragged output
-    counts, against the FP64 oracle."""
+@pytest.mark.parametrize("solver,n,p,zd", [(0, 200, 64, False), (1, 130, 40, False),
+                                           (0, 90, 24, False), (1, 70, 17, False),
+                                           (0, 300, 5, False), (0, 96, 33, True),
+                                           (1, 64, 8, True)])
+def test_wide_state_outputs_against_fp64(solver, n, p, zd, mode, bound):
+    """Outputs of wide FP32 states (n > 64) and of FP64 states run as tiles of
+    64 elements x 16, 32 or 64 outputs on the FP64 matrix cores, K over the
+    states then the inputs (only the states when D is zero): each tile width,
+    ragged output counts and K slices, against the FP64 oracle."""
     s = synth.make_system(solver, n, 3, p, -6.0, -0.2, 1e-3, 1500, seed=900 + n)
+    if zd:
+        s.D[:] = 0.0   # the oracle's D; the solver gets none (ZeroD)
     U = synth.samples_f64(s)
     api.set_precision(mode)
     try:
-        tv, y, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, s.D, U, 0.0, s.t_end, s.h,
-                                            s.x0)
+        tv, y, e = api.simulate_ode_sampled(solver, s.A, s.B, s.C, None if zd else s.D, U, 0.0,
+                                            s.t_end, s.h, s.x0)
     finally:
         api.set_precision(api.PLAIN32)
     assert e == 0, api.last_error_message()
