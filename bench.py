@@ -186,16 +186,18 @@ def cpu_model():
 
 # ---------------------------------------------------------------------- legs
 def served_leg(args, torch, api, systems, dev):
-    """All systems submitted at once to the server; device outputs."""
+    """All systems handed to the server in one batch submission; device outputs."""
     need = sum(api.Server.footprint(s.method, s.n, s.m, s.p, s.steps) for s in systems)
     srv = api.Server(lanes=args.lanes, device_bytes=int(need * 1.1) + (1 << 30),
                      pinned_bytes=1 << 30, precision=api.MIXED)
     outs = [torch.empty((s.steps, s.p), dtype=torch.float32, device=dev) for s in systems]
     lat = []
 
+    items = [(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h, o.data_ptr(), s.ucoef)
+             for s, o in zip(systems, outs)]
+
     def one_pass():
-        tk = [srv.submit_device(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h, o.data_ptr(),
-                                coef=s.ucoef) for s, o in zip(systems, outs)]
+        tk = srv.submit_device_batch(items)
         for t in tk:
             st, l, q = srv.wait(t)
             assert st == 0, st
@@ -448,13 +450,23 @@ def cpu_baseline_leg(systems, target_s=20.0):
         cpu(s.A, s.B, s.C, s.D, s.x0, U, pts, s.h)
         return time.perf_counter() - t0
 
-    # probe: 64 points of every system, then scale the prefix to about target_s
+    # probe: 64 points of every system; then the prefix length whose modelled
+    # cost (n^2 per point and system, the probe's rate) is about target_s
     probe = 64
+    cost = lambda q: sum(s.n * s.n * min(q, s.steps) for s in systems)
     with ThreadPoolExecutor(max_workers=threads) as ex:
         t0 = time.perf_counter()
         list(ex.map(lambda s: run(s, min(probe, s.steps)), systems))
         dt = time.perf_counter() - t0
-    pts = int(max(probe, min(1 << 18, probe * target_s / max(dt, 1e-3))))
+    rate = cost(probe) / max(dt, 1e-3)
+    lo, hi = probe, 1 << 18
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if cost(mid) / rate <= target_s:
+            lo = mid
+        else:
+            hi = mid - 1
+    pts = lo
     prefix = [min(pts, s.steps) for s in systems]
     with ThreadPoolExecutor(max_workers=threads) as ex:
         t0 = time.perf_counter()
@@ -483,6 +495,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--single-reps", type=int, default=20)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="copy BYTES once device to device and exit (calibrates the HBM "
+                         "byte counters against a copy of known size)")
     ap.add_argument("--only", choices=["group", "served", "plain", "single"], default=None,
                     help="run one leg alone (for profilers)")
     args = ap.parse_args()
@@ -495,6 +510,14 @@ def main():
     from parode_amd import api, synth
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
     systems = synth.batch_config(args.batch)
     total = float(sum(s.steps for s in systems))
     weighted = float(sum(s.steps * s.n for s in systems))

@@ -189,6 +189,21 @@ int ParODE_ServerSubmitDevice(void* server, int solver, int nStates, int nInputs
 							  const float* x0, const double* coef, double sigma,
 							  unsigned long long seed, const double* samples, float* dY,
 							  long* ticket);
+/* count device-output requests queued in one call, in order (element i of
+ * each array describes request i; coef, sigma, seed and samples may be NULL
+ * for all); tickets[i] receives request i's ticket.  For clients that hand
+ * over a batch of systems at once: the worker sees the whole batch when it
+ * next forms a group, and the per-request call overhead is paid once. */
+int ParODE_ServerSubmitDeviceBatch(void* server, int count, const int* solver,
+								   const int* nStates, const int* nInputs, const int* nOutputs,
+								   const int* zeroC, const int* zeroD, const long* points,
+								   const double* tStart, const double* tStep,
+								   const float* const* A, const float* const* B,
+								   const float* const* C, const float* const* D,
+								   const float* const* x0, const double* const* coef,
+								   const double* sigma, const unsigned long long* seed,
+								   const double* const* samples, float* const* dY,
+								   long* tickets);
 /* Blocks until the ticket completes; returns its status.  timesMs (may be
  * NULL) receives [submission -> completion, submission -> admission]. */
 int ParODE_ServerWait(void* server, long ticket, double* timesMs);

@@ -29,7 +29,7 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_ServerSubmitDevice", "ParODE_ServerGroupStats", "ParODE_GroupCreate",
                "ParODE_GroupExecute", "ParODE_GroupExecuteTimed", "ParODE_GroupStages",
                "ParODE_GroupLaunches", "ParODE_GroupFootprint", "ParODE_GroupEstimateBytes",
-               "ParODE_GroupDestroy"]
+               "ParODE_GroupDestroy", "ParODE_ServerSubmitDeviceBatch"]
 
 RK4, ABM = 0, 1
 PLAIN32, MIXED, FP64 = 0, 1, 2
@@ -125,6 +125,11 @@ def lib():
                                                 [_dp, _d, ctypes.c_ulonglong, _dp, _vp,
                                                  ctypes.POINTER(_l)])
         L.ParODE_ServerSubmitDevice.restype = _i32
+        _pv = ctypes.POINTER(_vp)
+        L.ParODE_ServerSubmitDeviceBatch.argtypes = (
+            [_vp, _i32] + [_ip] * 6 + [ctypes.POINTER(_l), _dp, _dp] + [_pv] * 6 +
+            [_dp, ctypes.POINTER(ctypes.c_ulonglong), _pv, _pv, ctypes.POINTER(_l)])
+        L.ParODE_ServerSubmitDeviceBatch.restype = _i32
         L.ParODE_ServerGroupStats.argtypes = [_vp, ctypes.POINTER(_l)]
         L.ParODE_ServerGroupStats.restype = _i32
         _pp = ctypes.POINTER(_vp)
@@ -458,6 +463,38 @@ class Server:
             raise ParODEError(r, "ParODE_ServerSubmitDevice")
         self._keep[t.value] = (sm, None)
         return t.value
+
+    def submit_device_batch(self, items):
+        """Queues many device-output solves in one call; items are tuples
+        (solver, A, B, C, D, x0, points, h, dY, coef) with dY a device address
+        of points x nOut floats and coef the m x 6 sinusoid coefficients.
+        Returns the tickets, in order."""
+        k = len(items)
+        ints = [np.zeros(k, np.int32) for _ in range(6)]
+        pts = np.zeros(k, np.int64)
+        t0s, hs = np.zeros(k), np.zeros(k)
+        ptr = [(ctypes.c_void_p * k)() for _ in range(7)]
+        keep = []
+        for i, (solver, A, B, C, D, x0, points, h, dY, coef) in enumerate(items):
+            A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
+            cf = None if coef is None else np.ascontiguousarray(coef, np.float64)
+            keep += [A, Bf, Cf, Df, x0f, cf]
+            for a, v in zip(ints, (solver, n, m, p, zC, zD)):
+                a[i] = v
+            pts[i], hs[i] = points, h
+            for a, v in zip(ptr, (A, Bf, Cf, Df, x0f, cf)):
+                a[i] = None if v is None else v.ctypes.data
+            ptr[6][i] = dY
+        tickets = (ctypes.c_long * k)()
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        r = lib().ParODE_ServerSubmitDeviceBatch(
+            self.handle, k, *(ip(a) for a in ints), pts.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+            dp(t0s), dp(hs), *ptr[:6], None, None, None, ptr[6], tickets)
+        if r != 0:
+            raise ParODEError(r, "ParODE_ServerSubmitDeviceBatch")
+        del keep
+        return list(tickets)
 
     def wait(self, ticket):
         ms = (ctypes.c_double * 2)()

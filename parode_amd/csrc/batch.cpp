@@ -294,6 +294,9 @@ const char* BatchSolve::stage_name(int stage)
     return stage >= 0 && stage < kStages ? names[stage] : "";
 }
 
+// work-groups a phase needs before its launches are spread over fan streams
+constexpr long kFanMinBlocks = 4096;
+
 void FanStreams::create()
 {
     for (int k = 0; k < kCount; ++k) {
@@ -342,8 +345,12 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
             size_t e = i + 1;
             while (e < refs_.size() && refs_[e].phase == refs_[i].phase) ++e;
             const int count = (int)(e - i);
-            const int used =
-                (f && wide(refs_[i].stage)) ? std::min(count - 1, FanStreams::kCount) : 0;
+            // spread only phases wide enough to fill the GPU several times over
+            // (a small group's fork and joins would cost more than its tails)
+            long blocks = 0;
+            for (size_t j = i; j < e; ++j) blocks += refs_[j].L->blocks();
+            const int used = (f && wide(refs_[i].stage) && blocks >= kFanMinBlocks)
+                                 ? std::min(count - 1, FanStreams::kCount) : 0;
             if (used > 0) {
                 PARODE_HIP(hipEventRecord(f->start, st));
                 for (int k = 0; k < used; ++k)

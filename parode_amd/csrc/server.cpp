@@ -122,12 +122,18 @@ Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
     pinned_.reset(new Arena(pinnedBytes, Arena::kPinnedHost));
     lanes_.resize(lanes);
     load_.assign(lanes, 0);
-    for (auto& s : lanes_) PARODE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     sides_.resize(lanes);
-    for (auto& sd : sides_) {
+    fans_.resize(lanes);
+    // a lane's streams are made one after another: the runtime hands the
+    // device's hardware queues to new streams in turn, so a lane's main, side
+    // and fan streams run on different queues
+    for (int l = 0; l < lanes; ++l) {
+        PARODE_HIP(hipStreamCreateWithFlags(&lanes_[l], hipStreamNonBlocking));
+        ScanCore::SideStream& sd = sides_[l];
         PARODE_HIP(hipStreamCreateWithFlags(&sd.stream, hipStreamNonBlocking));
         PARODE_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
         PARODE_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+        fans_[l].create();
     }
     st_.arenaBytes = arena_->capacity();
     worker_ = std::thread([this] { loop(); });
@@ -147,9 +153,10 @@ Server::~Server()
         if (sd.join) (void)hipEventDestroy(sd.join);
         if (sd.stream) (void)hipStreamDestroy(sd.stream);
     }
+    for (auto& f : fans_) f.destroy();
 }
 
-long Server::submit(const ServeRequest& r0)
+std::shared_ptr<Server::Job> Server::make_job(const ServeRequest& r0) const
 {
     auto j = std::make_shared<Job>();
     j->r = r0;
@@ -160,7 +167,9 @@ long Server::submit(const ServeRequest& r0)
                      (!r.zD && !r.D) || (!(r.zC && r.zD) && r.p <= 0) || !(r.h > 0.0);
     j->L = layout(r);
     const size_t n = r.n, m = r.m, p = r.p;
-    if (!bad) {
+    if (bad) {
+        j->status = ParODE_InvalidSystem;
+    } else {
         j->mats.assign(j->L.coef / 4, 0.0f);
         std::memcpy(&j->mats[j->L.A / 4], r.A, n * n * 4);
         if (m) std::memcpy(&j->mats[j->L.B / 4], r.B, n * m * 4);
@@ -172,26 +181,43 @@ long Server::submit(const ServeRequest& r0)
     j->r.A = j->r.B = j->r.C = j->r.D = j->r.x0 = nullptr;
     j->r.coef = nullptr;
     j->tSubmit = Clock::now();
-    std::lock_guard<std::mutex> g(mu_);
+    return j;
+}
+
+long Server::enqueue(const std::shared_ptr<Job>& j)
+{
     j->id = next_++;
     jobs_[j->id] = j;
     ++st_.submitted;
-    if (bad) {
-        j->status = ParODE_InvalidSystem;
-        j->tAdmit = j->tDone = j->tSubmit;
-        ++st_.failed;
-        return j->id;
-    }
-    const size_t stage = j->L.up + (r.yDevice ? 0 : j->L.yBytes);
-    if (footprint(r, precision_) > arena_->capacity() || stage > pinned_->capacity()) {
+    const size_t stage = j->L.up + (j->r.yDevice ? 0 : j->L.yBytes);
+    if (j->status == 1 &&
+        (footprint(j->r, precision_) > arena_->capacity() || stage > pinned_->capacity()))
         j->status = ParODE_NotEnoughResources;
+    if (j->status != 1) {
         j->tAdmit = j->tDone = j->tSubmit;
         ++st_.failed;
         return j->id;
     }
     queue_.push_back(j);
-    work_cv_.notify_one();
     return j->id;
+}
+
+long Server::submit(const ServeRequest& r)
+{
+    auto j = make_job(r);
+    std::lock_guard<std::mutex> g(mu_);
+    const long id = enqueue(j);
+    work_cv_.notify_one();
+    return id;
+}
+
+void Server::submit(const ServeRequest* rs, size_t count, long* tickets)
+{
+    std::vector<std::shared_ptr<Job>> js(count);
+    for (size_t i = 0; i < count; ++i) js[i] = make_job(rs[i]);
+    std::lock_guard<std::mutex> g(mu_);
+    for (size_t i = 0; i < count; ++i) tickets[i] = enqueue(js[i]);
+    work_cv_.notify_one();
 }
 
 bool Server::grouped(const Job& j) const
@@ -309,7 +335,8 @@ bool Server::start(Group& g)
     check_range("launch records", upTot + yTot, descBytes, stage->bytes);
     PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
     if (g.grouped) {
-        g.batch->run(s, stage->p + upTot + yTot, g.desc, true, nullptr, &sides_[g.lane]);
+        g.batch->run(s, stage->p + upTot + yTot, g.desc, true, nullptr, &sides_[g.lane],
+                     &fans_[g.lane]);
     } else {
         const Job& j = *g.jobs[0];
         const ServeRequest& r = j.r;
@@ -577,6 +604,50 @@ int ParODE_ServerSubmitDevice(void* server, int solver, int nStates, int nInputs
     r.yDevice = true;
     try {
         *ticket = static_cast<parode::Server*>(server)->submit(r);
+        return ParODE_OK;
+    } catch (std::exception& e) {
+        parode::last_error() = e.what();
+        return ParODE_NotEnoughResources;
+    }
+}
+
+int ParODE_ServerSubmitDeviceBatch(void* server, int count, const int* solver,
+                                   const int* nStates, const int* nInputs, const int* nOutputs,
+                                   const int* zeroC, const int* zeroD, const long* points,
+                                   const double* tStart, const double* tStep,
+                                   const float* const* A, const float* const* B,
+                                   const float* const* C, const float* const* D,
+                                   const float* const* x0, const double* const* coef,
+                                   const double* sigma, const unsigned long long* seed,
+                                   const double* const* samples, float* const* dY, long* tickets)
+{
+    if (!server || count < 0 || (count > 0 && (!solver || !nStates || !nInputs || !nOutputs ||
+                                               !zeroC || !zeroD || !points || !tStart ||
+                                               !tStep || !A || !B || !C || !D || !x0 || !dY ||
+                                               !tickets)))
+        return ParODE_InvalidSystem;
+    std::vector<parode::ServeRequest> rs((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        parode::ServeRequest& r = rs[(size_t)i];
+        r.solver = solver[i];
+        r.n = nStates[i];
+        r.m = nInputs[i];
+        r.p = nOutputs[i];
+        r.zC = zeroC[i] != 0;
+        r.zD = zeroD[i] != 0;
+        r.points = points[i];
+        r.t0 = tStart[i];
+        r.h = tStep[i];
+        r.A = A[i]; r.B = B[i]; r.C = C[i]; r.D = D[i]; r.x0 = x0[i];
+        r.coef = coef ? coef[i] : nullptr;
+        r.sigma = sigma ? sigma[i] : 0.0;
+        r.seed = seed ? seed[i] : 0;
+        r.samples = samples ? samples[i] : nullptr;
+        r.Y = dY[i];
+        r.yDevice = true;
+    }
+    try {
+        static_cast<parode::Server*>(server)->submit(rs.data(), rs.size(), tickets);
         return ParODE_OK;
     } catch (std::exception& e) {
         parode::last_error() = e.what();

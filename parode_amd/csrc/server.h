@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "hip_util.h"
+#include "batch.h"
 #include "scan_core.h"
 
 namespace parode {
@@ -62,6 +63,8 @@ public:
     // Returns a ticket (> 0); copies the matrices and coefficients, so the
     // caller keeps only `samples` and `Y` alive until the ticket completes.
     long submit(const ServeRequest& r);
+    // The same for count requests, queued together (tickets[i] for rs[i]).
+    void submit(const ServeRequest* rs, size_t count, long* tickets);
     // Blocks until the ticket completes; returns its status (0 or an
     // ErrorCode) and the times from submission to completion and to admission.
     int wait(long ticket, double* latencyMs, double* queueMs);
@@ -76,6 +79,10 @@ private:
     using Clock = std::chrono::steady_clock;
     void loop();
     bool grouped(const Job& j) const;
+    // the job of a request (its matrices copied); status InvalidSystem when bad
+    std::shared_ptr<Job> make_job(const ServeRequest& r) const;
+    // gives j its ticket and queues it (or fails it); mu_ held
+    long enqueue(const std::shared_ptr<Job>& j);
     // enqueues the group's work on lanes_[lane]; false: no memory for it now
     bool start(Group& g);
     void finish(Group& g, int status);
@@ -84,6 +91,7 @@ private:
     size_t groupCap_ = 1024;   // requests per group (halved while memory is short)
     std::vector<hipStream_t> lanes_;
     std::vector<ScanCore::SideStream> sides_;   // per lane: the grouped solves' side stream
+    std::vector<FanStreams> fans_;              // per lane: the grouped solves' wide phases
     std::vector<int> load_;
     std::unique_ptr<Arena> arena_, pinned_;
     std::mutex mu_;

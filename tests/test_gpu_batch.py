@@ -123,3 +123,36 @@ def test_device_outputs_equal_host_outputs():
         srv.close()
     for s, y, o in zip(systems, ys, outs):
         np.testing.assert_array_equal(o.cpu().numpy(), y, err_msg=f"n={s.n} N={s.steps}")
+
+
+def test_batch_submission_equals_single_submissions():
+    """ParODE_ServerSubmitDeviceBatch queues a whole batch in one call: every
+    request's outputs equal those of the same request submitted on its own
+    (bit for bit), and the tickets come back in order."""
+    systems = _systems()[:12]
+    need = sum(api.Server.footprint(s.method, s.n, s.m, s.p, s.steps) for s in systems)
+    dev = torch.device("cuda:0")
+    srv = api.Server(lanes=2, device_bytes=int(need * 1.2) + (256 << 20), pinned_bytes=64 << 20,
+                     precision=api.MIXED)
+    one = [torch.full((s.steps, s.p), float("nan"), dtype=torch.float32, device=dev)
+           for s in systems]
+    many = [torch.full_like(o, float("nan")) for o in one]
+    try:
+        single = [srv.submit_device(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h,
+                                    o.data_ptr(), coef=s.ucoef) for s, o in zip(systems, one)]
+        for t in single:
+            assert srv.wait(t)[0] == 0
+        tickets = srv.submit_device_batch([(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h,
+                                            o.data_ptr(), s.ucoef)
+                                           for s, o in zip(systems, many)])
+        assert tickets == sorted(tickets) and len(set(tickets)) == len(systems)
+        for t in tickets:
+            assert srv.wait(t)[0] == 0
+        gs = srv.group_stats()
+    finally:
+        srv.close()
+    assert gs["largest_group"] == len(systems)   # the batch formed one group
+    for s, a, b in zip(systems, one, many):
+        np.testing.assert_array_equal(b.cpu().numpy(), a.cpu().numpy(),
+                                      err_msg=f"n={s.n} N={s.steps}")
+
