@@ -52,6 +52,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
+_F32, _F64 = np.dtype(np.float32), np.dtype(np.float64)
 
 
 def lib():
@@ -470,21 +471,41 @@ class Server:
         of points x nOut floats and coef the m x 6 sinusoid coefficients.
         Returns the tickets, in order."""
         k = len(items)
-        ints = [np.zeros(k, np.int32) for _ in range(6)]
-        pts = np.zeros(k, np.int64)
-        t0s, hs = np.zeros(k), np.zeros(k)
-        ptr = [(ctypes.c_void_p * k)() for _ in range(7)]
+        ints = [[0] * k for _ in range(6)]
+        pts, hs = [0] * k, [0.0] * k
+        ptr = [[None] * k for _ in range(7)]
         keep = []
+
+        def addr(a, dtype):
+            # the array's address, converted (and kept alive) only when needed
+            if a is None:
+                return None, None
+            if not (type(a) is np.ndarray and a.dtype is dtype and a.flags.c_contiguous):
+                a = np.ascontiguousarray(a, dtype)
+                keep.append(a)
+            if a.size == 0:
+                return None, None
+            return a, a.ctypes.data
+
         for i, (solver, A, B, C, D, x0, points, h, dY, coef) in enumerate(items):
-            A, Bf, Cf, Df, x0f, n, m, p, zB, zC, zD = _system_args(A, B, C, D, x0)
-            cf = None if coef is None else np.ascontiguousarray(coef, np.float64)
-            keep += [A, Bf, Cf, Df, x0f, cf]
+            A, pA = addr(A, _F32)
+            B, pB = addr(B, _F32)
+            C, pC = addr(C, _F32)
+            D, pD = addr(D, _F32)
+            _, px = addr(x0, _F32)
+            _, pc = addr(coef, _F64)
+            n = A.shape[0]
+            m = B.shape[1] if B is not None else (D.shape[1] if D is not None else 0)
+            zC, zD = int(C is None), int(D is None)
+            p = n if (zC and zD) else (C.shape[0] if C is not None else D.shape[0])
             for a, v in zip(ints, (solver, n, m, p, zC, zD)):
                 a[i] = v
             pts[i], hs[i] = points, h
-            for a, v in zip(ptr, (A, Bf, Cf, Df, x0f, cf)):
-                a[i] = None if v is None else v.ctypes.data
-            ptr[6][i] = dY
+            for a, v in zip(ptr, (pA, pB, pC, pD, px, pc, dY)):
+                a[i] = v
+        ints = [np.array(a, np.int32) for a in ints]
+        pts, t0s, hs = np.array(pts, np.int64), np.zeros(k), np.array(hs, np.float64)
+        ptr = [(ctypes.c_void_p * k)(*a) for a in ptr]
         tickets = (ctypes.c_long * k)()
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -3,6 +3,7 @@
 #include <ParODEEnums.h>
 
 #include <algorithm>
+#include <exception>
 #include <cstring>
 
 #include "batch.h"
@@ -43,6 +44,22 @@ Layout layout(const ServeRequest& r)
     L.up = up(o);
     L.yBytes = up((size_t)r.points * L.nOut * 4);
     return L;
+}
+
+// Runs fn(0) .. fn(parts - 1), parts - 1 of them on helper threads (inline
+// when a thread cannot be started); fn must not throw.
+template <typename F>
+void run_parts(size_t parts, F&& fn)
+{
+    std::vector<std::thread> helpers;
+    size_t p = 1;
+    try {
+        for (; p < parts; ++p) helpers.emplace_back(fn, p);
+    } catch (...) {
+        for (; p < parts; ++p) fn(p);
+    }
+    fn(0);
+    for (auto& t : helpers) t.join();
 }
 
 // Throws unless [off, off + bytes) lies inside a block of blockBytes.
@@ -214,7 +231,19 @@ long Server::submit(const ServeRequest& r)
 void Server::submit(const ServeRequest* rs, size_t count, long* tickets)
 {
     std::vector<std::shared_ptr<Job>> js(count);
-    for (size_t i = 0; i < count; ++i) js[i] = make_job(rs[i]);
+    // a large batch's copies of the matrices split among a few threads
+    const size_t parts = std::min<size_t>(8, count / 64 + 1);
+    std::vector<std::exception_ptr> err(parts);
+    auto copy = [&](size_t p) {
+        try {
+            for (size_t i = p; i < count; i += parts) js[i] = make_job(rs[i]);
+        } catch (...) {
+            err[p] = std::current_exception();
+        }
+    };
+    run_parts(parts, copy);
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
     std::lock_guard<std::mutex> g(mu_);
     for (size_t i = 0; i < count; ++i) tickets[i] = enqueue(js[i]);
     work_cv_.notify_one();
@@ -311,20 +340,17 @@ bool Server::start(Group& g)
         return false;
     }
     stage->arena = pinned_.get();
-    // every host copy and every transfer stays inside its own block
+    // every host copy and every transfer stays inside its own block (checked
+    // for all requests before any copy); a large group's copies into the
+    // pinned stage are split among a few threads
     for (size_t i = 0; i < g.jobs.size(); ++i) {
         Job& j = *g.jobs[i];
         const ServeRequest& r = j.r;
         check_range("staged matrices", upOff[i], j.mats.size() * 4, upTot);
         check_range("staged coefficients", upOff[i] + j.L.coef, j.coef.size() * 8, upTot);
-        std::memcpy(stage->p + upOff[i], j.mats.data(), j.mats.size() * 4);
-        if (!j.coef.empty())
-            std::memcpy(stage->p + upOff[i] + j.L.coef, j.coef.data(), j.coef.size() * 8);
-        if (r.samples && r.m) {
-            const size_t ub = (size_t)(2 * r.points + 1) * r.m * 8;
-            check_range("staged samples", upOff[i] + j.L.U, ub, upTot);
-            std::memcpy(stage->p + upOff[i] + j.L.U, r.samples, ub);
-        }
+        if (r.samples && r.m)
+            check_range("staged samples", upOff[i] + j.L.U, (size_t)(2 * r.points + 1) * r.m * 8,
+                        upTot);
         if (!r.yDevice) {
             check_range("device outputs", upTot + uTot + yOff[i], j.L.yBytes, ioBytes);
             check_range("staged outputs", upTot + yOff[i], j.L.yBytes, upTot + yTot);
@@ -332,6 +358,20 @@ bool Server::start(Group& g)
         j.stage = stage;
         j.stY = upTot + yOff[i];
     }
+    const size_t count = g.jobs.size(), parts = std::min<size_t>(8, count / 64 + 1);
+    auto copy = [&](size_t p) {
+        for (size_t i = p; i < count; i += parts) {
+            const Job& j = *g.jobs[i];
+            const ServeRequest& r = j.r;
+            std::memcpy(stage->p + upOff[i], j.mats.data(), j.mats.size() * 4);
+            if (!j.coef.empty())
+                std::memcpy(stage->p + upOff[i] + j.L.coef, j.coef.data(), j.coef.size() * 8);
+            if (r.samples && r.m)
+                std::memcpy(stage->p + upOff[i] + j.L.U, r.samples,
+                            (size_t)(2 * r.points + 1) * r.m * 8);
+        }
+    };
+    run_parts(parts, copy);
     check_range("launch records", upTot + yTot, descBytes, stage->bytes);
     PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
     if (g.grouped) {
