@@ -3394,13 +3394,15 @@ void pad_cast(hipStream_t s, int rows, int cols, int pp, const float* in, double
 
 // Outputs as tiles of 64 elements x 16 JT outputs of the product
 // Y[e][j] = sum_k [x_e | u_e][k] [C^T ; D^T][k][j] on the FP64 matrix cores
-// (v_mfma_f64_16x16x4f64; the FP32 states convert exactly): four waves, each
-// 16 elements x JT output tiles (one A fragment feeds JT products), K = the n
-// states (with C) then the m inputs (with D) in slices of 16 staged in LDS
-// and double buffered.  Elements past N read zero and are not stored;
-// output columns past pp (the padding of C^T and D^T) read zero.  Grid: x =
-// output tile, y = element tile.
-constexpr int kOG = 64, kOGKB = 16;
+// (v_mfma_f64_16x16x4f64): four waves, each 16 elements x JT output tiles (one
+// A fragment feeds JT products).  K runs over the n states (with C) in slices
+// of KB staged in LDS in the states' own type (FP32 states convert exactly at
+// the product; a slice of 32 FP32 states costs the LDS of 16 FP64 ones, twice
+// the bytes in flight per load), then over the m inputs (with D) in FP64
+// slices of 16; both double buffered.  Elements past N read zero and are not
+// stored; output columns past pp (the padding of C^T and D^T) read zero.
+// Grid: x = output tile, y = element tile.
+constexpr int kOG = 64, kOGU = 16;
 
 template <typename TX, typename TU, int JT>
 __device__ __forceinline__ void d_outputs_gemm(const VBlock vb, int n, int m, int p, int pp,
@@ -3412,63 +3414,121 @@ __device__ __forceinline__ void d_outputs_gemm(const VBlock vb, int n, int m, in
                                                float* __restrict__ Y)
 {
     constexpr int OW = 16 * JT;                      // outputs per tile
-    constexpr int BQ = OW / 16;                      // B values per thread and slice
-    __shared__ double As[2][kOG][kOGKB + 1];
-    __shared__ double Bs[2][kOGKB][OW + 1];
+    constexpr int KB = sizeof(TX) == 4 ? 32 : 16;    // states per slice
+    constexpr int AQ = KB / 4;                       // A values per thread and slice
+    constexpr int BQ = KB * OW / 256;                // C^T values per thread and slice
+    constexpr int BQU = OW / 16;                     // D^T values per thread and slice
+    // the states' slices, then (the same LDS) the inputs' slices
+    constexpr int AXW = (2 * kOG * (KB + 1) * (int)sizeof(TX) + 7) / 8;   // in doubles
+    constexpr int X_WORDS = AXW + 2 * KB * (OW + 1);
+    constexpr int U_WORDS = 2 * kOG * (kOGU + 1) + 2 * kOGU * (OW + 1);
+    __shared__ double smem[X_WORDS > U_WORDS ? X_WORDS : U_WORDS];
+    TX(*Ax)[kOG][KB + 1] = reinterpret_cast<TX(*)[kOG][KB + 1]>(smem);
+    double(*Bx)[KB][OW + 1] = reinterpret_cast<double(*)[KB][OW + 1]>(smem + AXW);
+    double(*Au)[kOG][kOGU + 1] = reinterpret_cast<double(*)[kOG][kOGU + 1]>(smem);
+    double(*Bu)[kOGU][OW + 1] =
+        reinterpret_cast<double(*)[kOGU][OW + 1]>(smem + 2 * kOG * (kOGU + 1));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, q4 = lane >> 4;
     const long e0 = vb.y * kOG;
     const int j0 = (int)vb.x * OW;
-    const int kC = useC ? n : 0, K = kC + (useD ? m : 0);
-    const int ar = tid >> 2, ak = (tid & 3) * 4;     // A slice: element, first of four k
-    const int bk = tid >> 4, bj = (tid & 15) * BQ;   // B slice: k, first of BQ outputs
+    const int kC = useC ? n : 0, kU = useD ? m : 0;
+    const int ar = tid >> 2;                         // A slice: element
     const long ea = e0 + ar;
     const bool realA = ea < N;
     const TX* xe = X + ea * na + (long)oX * n;
     const TU* ue = U + (ea * sO + oO) * m;
-    double ra[4], rb[BQ];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = k0 + ak + q;
-            ra[q] = (realA && k < K) ? (k < kC ? (double)xe[k] : (double)ue[k - kC]) : 0.0;
-        }
-        const int kb = k0 + bk;
-#pragma unroll
-        for (int q = 0; q < BQ; ++q) {
-            const int j = j0 + bj + q;
-            rb[q] = (kb < K && j < pp) ? (kb < kC ? CT[(long)kb * pp + j]
-                                                  : DT[(long)(kb - kC) * pp + j])
-                                       : 0.0;
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) As[buf][ar][ak + q] = ra[q];
-#pragma unroll
-        for (int q = 0; q < BQ; ++q) Bs[buf][bk][bj + q] = rb[q];
-    };
     doublex4 acc[JT];
 #pragma unroll
     for (int t = 0; t < JT; ++t) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
-    load(0);
-    store(0);
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < K; k0 += kOGKB) {
-        const bool more = k0 + kOGKB < K;
-        if (more) load(k0 + kOGKB);
+    // C x: states in slices of KB
+    if (kC > 0) {
+        const int ak = (tid & 3) * AQ;
+        const int bk = tid / (OW / BQ), bj = (tid % (OW / BQ)) * BQ;
+        TX ra[AQ];
+        double rb[BQ];
+        auto load = [&](int k0) {
 #pragma unroll
-        for (int ks = 0; ks < kOGKB; ks += 4) {
-            const double a = As[buf][wave * 16 + l16][ks + q4];
+            for (int q = 0; q < AQ; ++q) {
+                const int k = k0 + ak + q;
+                ra[q] = (realA && k < kC) ? xe[k] : (TX)0;
+            }
+            const int kb = k0 + bk;
 #pragma unroll
-            for (int t = 0; t < JT; ++t)
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Bs[buf][ks + q4][t * 16 + l16],
-                                                             acc[t], 0, 0, 0);
-        }
-        if (more) store(buf ^ 1);
+            for (int q = 0; q < BQ; ++q) {
+                const int j = j0 + bj + q;
+                rb[q] = (kb < kC && j < pp) ? CT[(long)kb * pp + j] : 0.0;
+            }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < AQ; ++q) Ax[buf][ar][ak + q] = ra[q];
+#pragma unroll
+            for (int q = 0; q < BQ; ++q) Bx[buf][bk][bj + q] = rb[q];
+        };
+        load(0);
+        store(0);
         __syncthreads();
-        buf ^= 1;
+        int buf = 0;
+        for (int k0 = 0; k0 < kC; k0 += KB) {
+            const bool more = k0 + KB < kC;
+            if (more) load(k0 + KB);
+#pragma unroll
+            for (int ks = 0; ks < KB; ks += 4) {
+                const double a = (double)Ax[buf][wave * 16 + l16][ks + q4];
+#pragma unroll
+                for (int t = 0; t < JT; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        a, Bx[buf][ks + q4][t * 16 + l16], acc[t], 0, 0, 0);
+            }
+            if (more) store(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    // D u: inputs in FP64 slices of 16
+    if (kU > 0) {
+        const int ak = (tid & 3) * 4;
+        const int bk = tid >> 4, bj = (tid & 15) * BQU;
+        double ra[4], rb[BQU];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = k0 + ak + q;
+                ra[q] = (realA && k < kU) ? (double)ue[k] : 0.0;
+            }
+            const int kb = k0 + bk;
+#pragma unroll
+            for (int q = 0; q < BQU; ++q) {
+                const int j = j0 + bj + q;
+                rb[q] = (kb < kU && j < pp) ? DT[(long)kb * pp + j] : 0.0;
+            }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Au[buf][ar][ak + q] = ra[q];
+#pragma unroll
+            for (int q = 0; q < BQU; ++q) Bu[buf][bk][bj + q] = rb[q];
+        };
+        load(0);
+        store(0);
+        __syncthreads();
+        int buf = 0;
+        for (int k0 = 0; k0 < kU; k0 += kOGU) {
+            const bool more = k0 + kOGU < kU;
+            if (more) load(k0 + kOGU);
+#pragma unroll
+            for (int ks = 0; ks < kOGU; ks += 4) {
+                const double a = Au[buf][wave * 16 + l16][ks + q4];
+#pragma unroll
+                for (int t = 0; t < JT; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        a, Bu[buf][ks + q4][t * 16 + l16], acc[t], 0, 0, 0);
+            }
+            if (more) store(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
     // D[i][j] (element i, output j) is in lane j + 16 (i % 4), register i / 4
 #pragma unroll
