@@ -239,7 +239,7 @@ enum Kind {
     kChunksTwo160, kChunksTwo192, kChunksTwo224, kChunksTwo256,
     kDense16, kDense32, kDense48, kDense64, kDense96, kDense128, kDense160, kDense192, kDense224,
     kDense256,
-    kLevel32, kLevel32W, kLevel32Delta, kLevel32DeltaW, kLevel64, kGather, kTopMfma, kTopSimt, kRoot,
+    kLevel32, kLevel32W, kLevel32Delta, kLevel32DeltaW, kLevelG, kLevelGDelta, kLevel64, kGather, kTopMfma, kTopSimt, kRoot,
     kOutputsLanes, kOutputsKc8, kOutputsKc16, kOutputsKc24, kOutputsKc32, kOutputsMfma
 };
 struct Launch {

@@ -2631,6 +2631,103 @@ __device__ __forceinline__ void d_level_ks(const VBlock vb, int na, int ldp, lon
     }
 }
 
+// FP32 levels with many pairs as 64 x 64 tiles (64 pairs x 64 table rows per
+// work-group; four waves, each a 32 x 32 quadrant on v_mfma_f32_32x32x2f32),
+// K (the source states) in slices of 32 staged in LDS and double buffered:
+// the pairs' source rows and the table's rows arrive in coalesced runs, where
+// the K-split tiles gather every source value with its own strided load.
+// Sources, tables and the epilogue are those of d_level_ks (up: X[R] += P
+// X[L]; down from S, the copy of the old X[R]; DELTA tables P - I; Sn, the
+// next lower level's copy).  Grid: x = pair tile, y = row tile.
+constexpr int kLvG = 64, kLvKB = 32;
+
+template <bool DOWN, bool DELTA>
+__device__ __forceinline__ void d_level_gemm(const VBlock vb, int na, int ldp, long d, long np,
+                                             const float* __restrict__ PT,
+                                             const float* __restrict__ S, float* __restrict__ X,
+                                             float* __restrict__ Sn, long npn)
+{
+    __shared__ float As[2][kLvG][kLvKB + 1];
+    __shared__ float Bs[2][kLvKB][kLvG + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const long c0 = (long)vb.x * kLvG;
+    const int i0 = (int)vb.y * kLvG;
+    const int ar = tid >> 2, ak = (tid & 3) * 8;     // A slice: pair, first of eight states
+    const int bk = tid >> 3, bj = (tid & 7) * 8;     // B slice: state, first of eight rows
+    const long colA = (c0 + ar) < np ? c0 + ar : np - 1;
+    const float* src = DOWN ? S + colA * na : X + ((colA + 1) * 2 * d - 1 - d) * na;
+    float ra[8], rb[8];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + ak + q;
+            ra[q] = k < na ? src[k] : 0.0f;
+            const int kb = k0 + bk, i = i0 + bj + q;
+            rb[q] = (kb < na && i < ldp) ? PT[(long)kb * ldp + i] : 0.0f;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            As[buf][ar][ak + q] = ra[q];
+            Bs[buf][bk][bj + q] = rb[q];
+        }
+    };
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    const int l32 = lane & 31, h = lane >> 5;
+    for (int k0 = 0; k0 < na; k0 += kLvKB) {
+        const bool more = k0 + kLvKB < na;
+        if (more) load(k0 + kLvKB);
+#pragma unroll
+        for (int ks = 0; ks < kLvKB; ks += 2) {
+            // A(c = l%32, k = l/32), B(k = l/32, i = l%32)
+            const float a = As[buf][wr * 32 + l32][ks + h];
+            const float b = Bs[buf][ks + h][wc * 32 + l32];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // D: c = 8 (v / 4) + 4 (l / 32) + v % 4, i = l % 32 within the quadrant;
+    // all loads first (clamped addresses), then the guarded stores
+    const int i = i0 + wc * 32 + l32, icl = i < na ? i : na - 1;
+    float xo[16], xs[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const long c = c0 + wr * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        const long col = c < np ? c : np - 1;
+        const long R = (col + 1) * 2 * d - 1;
+        xo[v] = X[(DOWN ? R - d : R) * na + icl];
+        xs[v] = DOWN ? S[col * na + icl] : DELTA ? X[(R - d) * na + icl] : 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const float sum = acc[v];
+        const long col = c0 + wr * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        if (col >= np || i >= na) continue;
+        const long R = (col + 1) * 2 * d - 1;
+        if (!DOWN) {
+            X[R * na + i] = DELTA ? xo[v] + (xs[v] + sum) : xo[v] + sum;
+        } else {
+            const float y = DELTA ? xo[v] + (xs[v] + sum) : sum + xo[v];
+            X[(R - d) * na + i] = xs[v];
+            X[R * na + i] = y;
+            if (Sn) {
+                if (2 * col < npn) Sn[(2 * col) * na + i] = xs[v];
+                if (2 * col + 1 < npn) Sn[(2 * col + 1) * na + i] = y;
+            }
+        }
+    }
+}
+
 template <typename TT, typename TX, bool DOWN, bool DELTA, bool SMALL>
 __global__ __launch_bounds__(256) void k_level_ks(int na, int ldp, long d, long np,
                                                   const TT* __restrict__ PT,
@@ -4206,6 +4303,15 @@ __global__ __launch_bounds__(256) void kg_level(const Level* __restrict__ args,
                                               DOWN ? a.Sn : nullptr, DOWN ? a.npn : 0L);
 }
 
+template <bool DOWN, bool DELTA>
+__global__ __launch_bounds__(256) void kg_level_gemm(const Level* __restrict__ args,
+                                                     const unsigned* __restrict__ st, int ns)
+{
+    PARODE_GROUP(Level);
+    d_level_gemm<DOWN, DELTA>(vb, a.na, a.ldp, a.d, a.np, static_cast<const float*>(a.PT), a.S,
+                              a.X, DOWN ? a.Sn : nullptr, DOWN ? a.npn : 0L);
+}
+
 __global__ __launch_bounds__(256) void kg_gather(const Gather* __restrict__ args,
                                                  const unsigned* __restrict__ st, int ns)
 {
@@ -4514,6 +4620,9 @@ bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nP
     return true;
 }
 
+// pairs from which a grouped FP32 level takes the LDS-staged 64 x 64 tiles
+constexpr long kLevelGemmPairs = 256;
+
 void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long d, long np,
                const void* PT, float* S, float* X, float* Sn, long npn)
 {
@@ -4521,6 +4630,13 @@ void add_level(Phase& ph, bool down, bool f64, bool delta, int na, int ldp, long
     if (!f64 && ldp % 32 != 0) throw std::invalid_argument("add_level: FP32 table must be padded");
     if (down && (!S || (Sn && (Sn == S || npn < 0 || npn > 2 * np))))
         throw std::invalid_argument("add_level: bad copy buffers");
+    // FP32 with many pairs: 64 x 64 tiles staged in LDS
+    if (!f64 && np >= kLevelGemmPairs && na >= 64) {
+        const long gx = (np + kLvG - 1) / kLvG, gy = (na + kLvG - 1) / kLvG;
+        append(ph, delta ? kLevelGDelta : kLevelG, down,
+               Level{na, ldp, d, np, PT, S, X, Sn, Sn ? npn : 0, gx}, gx * gy);
+        return;
+    }
     // FP32: 32 x 32 tiles (half the table and state traffic per product) once
     // the system's own grid of them has 64 work-groups, else 16 x 16
     const bool wide = !f64 && ((np + 31) / 32) * ((na + 31) / 32) >= 64;
@@ -4646,6 +4762,8 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     case kLevel32W: PARODE_RUN2((kg_level<float, false, false, false>), (kg_level<float, true, false, false>), Level, 256); break;
     case kLevel32Delta: PARODE_RUN2((kg_level<float, false, true, true>), (kg_level<float, true, true, true>), Level, 256); break;
     case kLevel32DeltaW: PARODE_RUN2((kg_level<float, false, true, false>), (kg_level<float, true, true, false>), Level, 256); break;
+    case kLevelG: PARODE_RUN2((kg_level_gemm<false, false>), (kg_level_gemm<true, false>), Level, 256); break;
+    case kLevelGDelta: PARODE_RUN2((kg_level_gemm<false, true>), (kg_level_gemm<true, true>), Level, 256); break;
     case kLevel64: PARODE_RUN2((kg_level<double, false, false, false>), (kg_level<double, true, false, false>), Level, 256); break;
     case kGather: PARODE_RUN(kg_gather, Gather, 256); break;
     case kTopMfma: PARODE_RUN(kg_top<true>, Top, kTopThreads); break;
