@@ -187,20 +187,29 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         }
         // leaves (BTimesURK4thOrder.cl / BTimesUAdamsMoulton.cl)
         // (matrix cores: [B | AB] factored for ABM, B-bar for RK4; else the vector ALUs)
+        // Only the leaves up to the end of the last chunk holding a real element
+        // (of the last level-0 pair without chunks) are read: the chunk passes
+        // skip chunks past nScan and the tree levels pairs past it, so the
+        // padding's leaves beyond are neither formed nor zeroed.  The tree's
+        // aggregates over padding then hold stale values; no real element's
+        // prefix reads them (a prefix takes only left-sibling aggregates), and
+        // the total (tot) is not used by a grouped solve, which has no carry.
+        const long nLeaf = std::min(s.nPad, (s.nScan + (1L << std::max(s.lc, 1)) - 1) >>
+                                                std::max(s.lc, 1) << std::max(s.lc, 1));
         bool ok;
         if (abm) {
             k::LeafComb G;
             k::abm_leaf_factors(q.h, G);
             ok = k::g::leaves_fit(n, m, 5, 2, 2, 2)
-                     ? k::g::add_leaves(phases_[kLeaves], n, m, 5, 2, na, 3 * n, s.nScan, s.nPad,
+                     ? k::g::add_leaves(phases_[kLeaves], n, m, 5, 2, na, 3 * n, s.nScan, nLeaf,
                                         c.BfT64_.get(), G, q.U, 2, 0, 2, X, s.lc == 0)
                      : k::g::add_leaves_simt(phases_[kLeaves], n, m, 5, na, 3 * n, s.nScan,
-                                             s.nPad, c.BbT64_.get(), q.U, 2, 0, 2, X, s.lc == 0);
+                                             nLeaf, c.BbT64_.get(), q.U, 2, 0, 2, X, s.lc == 0);
         } else {
             ok = k::g::leaves_fit(n, m, 3, 0, 2, 1)
-                     ? k::g::add_leaves(phases_[kLeaves], n, m, 3, 0, na, 0, s.nScan, s.nPad,
+                     ? k::g::add_leaves(phases_[kLeaves], n, m, 3, 0, na, 0, s.nScan, nLeaf,
                                         c.BbT64_.get(), k::LeafComb{}, q.U, 2, 0, 1, X, true)
-                     : k::g::add_leaves_simt(phases_[kLeaves], n, m, 3, na, 0, s.nScan, s.nPad,
+                     : k::g::add_leaves_simt(phases_[kLeaves], n, m, 3, na, 0, s.nScan, nLeaf,
                                              c.BbT64_.get(), q.U, 2, 0, 1, X, true);
         }
         if (!ok) throw std::invalid_argument("BatchSolve: leaves do not fit the kernel");

@@ -2860,11 +2860,22 @@ __device__ __forceinline__ void top_update(bool down, int na, long d, long c0, i
 }
 
 // Update for na >= 16 on the FP64 matrix cores: Out[i][c] = sum_j P[i][j] S[c][j]
-// for the cc <= 16 staged columns, A = P from the transposed FP64 table (lane
+// for the cc staged columns, A = P from the transposed FP64 table (lane
 // l: i = 16 ib + l % 16, j = 4 k + l / 16) and B = S from LDS (pitch SP, odd).
-// Wave w owns the 16-row blocks ib = w, w + 16, ...; each wave issues the loads
-// of 8 k-steps before their MFMAs.
+// The (16-row block, 16-column tile) items are dealt to the waves in turn, so
+// that a narrow state (one to three row blocks) still gives every wave a tile;
+// each wave issues the loads of 8 k-steps before their MFMAs.
 constexpr int kTopMfmaCols = 16;
+
+// columns staged per pass of the matrix-core top: one 16-column tile per wave
+// over the state's 16-row blocks, within 48 KiB of LDS
+__host__ __device__ constexpr int top_mfma_cols(int na)
+{
+    const int nb = (na + 15) / 16, SP = na + 1;
+    const int tiles = nb >= kTopThreads / 64 ? 1 : (kTopThreads / 64) / nb;
+    const int cc = kTopMfmaCols * tiles;
+    return cc < 6144 / SP ? cc : 6144 / SP;
+}
 
 template <typename TX>
 __device__ __forceinline__ void top_update_mfma(bool down, int na, int SP, long d, long c0, int cc,
@@ -2873,8 +2884,9 @@ __device__ __forceinline__ void top_update_mfma(bool down, int na, int SP, long 
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l16 = lane & 15, q4 = lane >> 4;
-    const int nb = (na + 15) / 16, ks = (na + 3) / 4;
-    for (int ib = w; ib < nb; ib += kTopThreads / 64) {
+    const int nb = (na + 15) / 16, ks = (na + 3) / 4, ct = (cc + 15) / 16;
+    for (int item = w; item < nb * ct; item += kTopThreads / 64) {
+        const int ib = item % nb, cb = (item / nb) * 16;
         const int ia = ib * 16 + l16;
         doublex4 acc = {0.0, 0.0, 0.0, 0.0};
         constexpr int U = 8;
@@ -2884,14 +2896,14 @@ __device__ __forceinline__ void top_update_mfma(bool down, int na, int SP, long 
             for (int u = 0; u < U; ++u) {
                 const int j = (k + u) * 4 + q4;
                 a[u] = (j < na && ia < na) ? PT[(long)j * na + ia] : 0.0;
-                b[u] = (j < na && l16 < cc) ? Ssh[l16 * SP + j] : 0.0;
+                b[u] = (j < na && cb + l16 < cc) ? Ssh[(cb + l16) * SP + j] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
         }
         // D[i][c] is in lane c + 16 (i % 4), register i / 4
-        const int c = l16;
+        const int c = cb + l16;
         if (c >= cc) continue;
         const long R = (c0 + c + 1) * 2 * d - 1, Lp = R - d;
 #pragma unroll
@@ -2977,12 +2989,12 @@ template <typename TX>
 void top(hipStream_t s, int na, int l0, int L, int l64, long nReal, const float* PT32,
          const double* PT64, TX* X, const double* seed, double* tot)
 {
-    // columns per pass: one MFMA B tile (16) for na >= 16, else what the
-    // threads can hold (groups x kTopMaxCols); at most 48 KiB of LDS staged
+    // columns per pass: a 16-column MFMA tile per wave for na >= 16, else what
+    // the threads can hold (groups x kTopMaxCols); at most 48 KiB of LDS staged
     const int rowThreads = na < kTopThreads ? na : kTopThreads;
     const int groups = kTopThreads / rowThreads;
     const int SP = na >= 16 ? na + 1 : na;
-    int CC = na >= 16 ? std::min(kTopMfmaCols, 6144 / SP) : std::min(groups * kTopMaxCols, 6144 / na);
+    int CC = na >= 16 ? top_mfma_cols(na) : std::min(groups * kTopMaxCols, 6144 / na);
     if (CC < 1) CC = 1;
     size_t lds = (size_t)CC * SP * sizeof(double);
     if (na >= 16)
@@ -3911,14 +3923,22 @@ __device__ __forceinline__ void d_sample_family(const VBlock vb, int m, const do
                                 double* __restrict__ out)
 {
     const long total = S * m;
-    for (long i = (long)vb.x * blockDim.x + threadIdx.x; i < total;
-         i += (long)vb.gx * blockDim.x) {
-        const long sl = i / m;
-        const int j = (int)(i - sl * m);
+    long i = (long)vb.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    // (step, input) of the sample and of the grid stride: the loop advances
+    // them without a 64-bit division per sample
+    const long stride = (long)vb.gx * blockDim.x;
+    long sl = i / m;
+    int j = (int)(i - sl * m);
+    const long dS = stride / m;
+    const int dJ = (int)(stride - dS * m);
+    for (; i < total; i += stride) {
         const double* k = coef + 6 * j;
         const double t = t0 + (double)(s0 + sl) * dt;
         const double g = fma(k[4], t, k[5]);
-        double u = fma(k[3], t, fma(k[1], cos(g), fma(k[0], sin(g), k[2])));
+        double sg, cg;
+        sincos(g, &sg, &cg);   // one argument reduction for both
+        double u = fma(k[3], t, fma(k[1], cg, fma(k[0], sg, k[2])));
         if (sigma != 0.0) {
             // Box-Muller on two counter-keyed splitmix64 draws
             uint64_t st = seed * 0xD1B54A32D192ED03ull + (uint64_t)(s0 + sl) * 0x9E3779B97F4A7C15ull +
@@ -3928,6 +3948,12 @@ __device__ __forceinline__ void d_sample_family(const VBlock vb, int m, const do
             u += sigma * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
         }
         out[i] = u;
+        j += dJ;
+        sl += dS;
+        if (j >= m) {
+            j -= m;
+            ++sl;
+        }
     }
 }
 
@@ -4660,8 +4686,7 @@ void add_top(Phase& ph, int na, int l0, int L, int l64, long nReal, const float*
     const int rowThreads = na < kTopThreads ? na : kTopThreads;
     const int groups = kTopThreads / rowThreads;
     const int SP = na >= 16 ? na + 1 : na;
-    int CC = na >= 16 ? std::min(kTopMfmaCols, 6144 / SP)
-                      : std::min(groups * kTopMaxCols, 6144 / na);
+    int CC = na >= 16 ? top_mfma_cols(na) : std::min(groups * kTopMaxCols, 6144 / na);
     if (CC < 1) CC = 1;
     append(ph, na >= 16 ? kTopMfma : kTopSimt, false,
            Top{na, l0, L, l64, nReal, PT32, PT64, X, seed, tot, CC, 1}, 1,
