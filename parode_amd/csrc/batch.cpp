@@ -271,6 +271,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
                                abm ? 6 : 0, yScan))
             throw std::invalid_argument("BatchSolve: outputs do not fit the kernel");
     }
+    // the longest-running work-groups of every launch first
+    for (auto& ph : phases_)
+        for (auto& kv : ph) k::g::order_by_cost(kv.second);
     // pack every launch's records and block table
     auto align = [](size_t v) { return (v + 15) / 16 * 16; };
     size_t off = 0;

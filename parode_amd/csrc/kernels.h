@@ -248,10 +248,16 @@ struct Launch {
     size_t argSize = 0;                 // bytes of one argument record
     std::vector<unsigned char> args;    // nsys records
     std::vector<unsigned> start{0};     // nsys + 1: first block of each system
+    std::vector<float> cost;            // nsys: the run time of one of a system's blocks
+                                        // relative to the others' (0: unknown)
     size_t lds = 0;                     // dynamic LDS bytes (the largest system's)
     int nsys() const { return (int)start.size() - 1; }
     unsigned blocks() const { return start.back(); }
 };
+// Reorders a launch's systems by decreasing cost (stable), so that the
+// longest-running work-groups are dispatched first and the short ones fill the
+// compute units around them instead of forming the launch's tail.
+void order_by_cost(Launch& L);
 // key: (kind, flag + 2 x LDS size class)
 using Phase = std::map<std::pair<int, int>, Launch>;
 

@@ -4413,7 +4413,8 @@ int lds_class(size_t lds)
 }
 
 template <typename A>
-void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds = 0)
+void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds = 0,
+            float cost = 0.0f)
 {
     if (blocks < 1) return;
     Launch& L = slot<A>(ph, kind, flag, lds_class(lds));
@@ -4423,10 +4424,38 @@ void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds 
     const unsigned long next = (unsigned long)L.start.back() + (unsigned long)blocks;
     if (next > 0x7fffffffUL) throw std::length_error("grouped launch: too many blocks");
     L.start.push_back((unsigned)next);
+    L.cost.push_back(cost);
     L.lds = std::max(L.lds, lds);
 }
 
 }  // namespace
+
+void order_by_cost(Launch& L)
+{
+    const int ns = L.nsys();
+    if (ns < 2 || (int)L.cost.size() != ns || L.args.size() != (size_t)ns * L.argSize) return;
+    std::vector<int> idx(ns);
+    for (int i = 0; i < ns; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return L.cost[a] > L.cost[b]; });
+    bool same = true;
+    for (int i = 0; i < ns && same; ++i) same = idx[i] == i;
+    if (same) return;
+    std::vector<unsigned char> args(L.args.size());
+    std::vector<unsigned> start(1, 0u);
+    std::vector<float> cost;
+    start.reserve(ns + 1);
+    cost.reserve(ns);
+    for (int k = 0; k < ns; ++k) {
+        const int s = idx[k];
+        std::memcpy(args.data() + (size_t)k * L.argSize, L.args.data() + (size_t)s * L.argSize,
+                    L.argSize);
+        start.push_back(start.back() + (L.start[s + 1] - L.start[s]));
+        cost.push_back(L.cost[s]);
+    }
+    L.args.swap(args);
+    L.start.swap(start);
+    L.cost.swap(cost);
+}
 
 void add_cast(Phase& ph, long count, const float* in, double* out)
 {
@@ -4463,10 +4492,10 @@ void add_gemm_ld(Phase& ph, int r, int kk, int c, const double* X, int ldx, cons
     // 16 x 16 kernel's lower rate (about half)
     if (r >= 96 && c >= 96 && kk >= 64) {
         const long gx = (c + kG64 - 1) / kG64, gy = (r + kG64 - 1) / kG64;
-        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy);
+        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, (float)kk);
     } else {
         const long gx = (c + 15) / 16, gy = (r + 15) / 16;
-        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy);
+        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, (float)kk);
     }
 }
 
@@ -4571,13 +4600,14 @@ bool add_leaves(Phase& ph, int n, int m, int kq, int kg, int na, int off, long n
     // persistent work-groups: a few hundred per system (each walks its blocks,
     // the next block's samples loaded during the current one), so that the
     // per-work-group start-up is paid rarely
-    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 256), 1);
+    const long nb = (nPad + perBlock - 1) / perBlock;
+    const long b = std::max<long>(std::min<long>(nb, 256), 1);
     // without writeZeros the first reader (a chunk pass) reads only rows
     // [off, off + n): no other row is written for real leaves
     append(ph, kg > 0 ? kLeavesFac : kLeavesPlain, false,
            Leaves{n, m, kq, na, off, nReal, nPad, groups, pitch, magic, kg, G, writeZeros,
                   writeZeros ? off / 32 * 32 : off, Bf, U, sL, oL, rs, X, b},
-           b, lds);
+           b, lds, (float)((nb + b - 1) / b));
     return true;
 }
 
@@ -4599,11 +4629,12 @@ bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nRea
     while (groups > 1 && lds_for(groups) > 65536) groups >>= 1;
     if (lds_for(groups) > 65536) return false;
     const long perBlock = (long)groups * kLeafEB;
-    const long b = std::max<long>(std::min<long>((nPad + perBlock - 1) / perBlock, 256), 1);
+    const long nb = (nPad + perBlock - 1) / perBlock;
+    const long b = std::max<long>(std::min<long>(nb, 256), 1);
     append(ph, kLeavesSimt, false,
            Leaves{n, m, kq, na, off, nReal, nPad, groups, tpeLog, 0u, 0, LeafComb{}, writeZeros,
                   writeZeros ? off / 32 * 32 : off, BbT, U, sL, oL, rs, X, b},
-           b, lds_for(groups));
+           b, lds_for(groups), (float)((nb + b - 1) / b));
     return true;
 }
 
@@ -4613,7 +4644,8 @@ bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad,
     if (!abm_chunks_factored_supported(n) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 ||
         nReal > nPad)
         return false;
-    const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
+    // work-groups for the chunks that hold a real element (the others return at once)
+    const long b = (((nReal + (1L << lc) - 1) >> lc) + kChunkCols - 1) / kChunkCols;
     const Chunks a{n, lc, nReal, A64, A264, abm_chunk_factors(h), X, b};
     // padded state sizes: factored up to 128, two products above
     static const int kNp[] = {16, 32, 48, 64, 96, 128, 160, 192, 224, 256};
@@ -4626,7 +4658,7 @@ bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad,
     const size_t lds = np <= 128
                            ? (size_t)2 * kChunkCols * chunk_pitch(np / 2, 2 * np) * sizeof(float)
                            : (size_t)2 * kChunkCols * chunk_pitch(np / 4, np) * sizeof(float);
-    append(ph, kKind[t], pass2, a, b, lds);
+    append(ph, kKind[t], pass2, a, b, lds, (float)(1L << lc));
     return true;
 }
 
@@ -4635,14 +4667,15 @@ bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nP
 {
     if (!chunks_dense_supported(na) || lc < 1 || (nPad >> lc) < 1 || nReal < 1 || nReal > nPad)
         return false;
-    const long b = ((nPad >> lc) + kChunkCols - 1) / kChunkCols;
+    const long b = (((nReal + (1L << lc) - 1) >> lc) + kChunkCols - 1) / kChunkCols;
     static const int kNp[] = {16, 32, 48, 64, 96, 128, 160, 192, 224, 256};
     static const int kKind[] = {kDense16, kDense32, kDense48, kDense64, kDense96, kDense128,
                                 kDense160, kDense192, kDense224, kDense256};
     int t = 0;
     while (kNp[t] < na) ++t;
     append(ph, kKind[t], pass2, Chunks{na, lc, nReal, P64, nullptr, AbmFactors{}, X, b}, b,
-           (size_t)kChunkCols * chunk_pitch(kNp[t] / 4, kNp[t]) * sizeof(float));
+           (size_t)kChunkCols * chunk_pitch(kNp[t] / 4, kNp[t]) * sizeof(float),
+           (float)(1L << lc));
     return true;
 }
 
