@@ -46,7 +46,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
     ScanOptions opt;
     opt.precision = kMixed;
     // stages, in order; the squarings and the tree levels get one phase per level
-    enum { kIn = 0, kR1, kR2, kR3, kComb, kPost, kPow0 };
+    enum { kSmp = 0, kIn, kR1, kR2, kR3, kComb, kPost, kPow0 };
     int maxL = 0, upSteps = 0;
     struct Shape {
         long nScan, nPad;
@@ -81,7 +81,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
               kOut = kChunk2 + 1;
     phases_.resize(kOut + 1);
     auto stage_of = [&](int ph) {
-        if (ph == kIn) return kStageInputs;
+        if (ph <= kIn) return kStageInputs;
         if (ph <= kPost) return kStageMap;
         if (ph < kTables) return kStagePowers;
         if (ph == kTables) return kStageTables;
@@ -116,8 +116,8 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         // AdamsMoulton.cpp:238-316 take A and B; SystemOutput.cl:13-60 C and D)
         k::g::Phase& in = phases_[kIn];
         if (q.coef)
-            k::g::add_sample(in, m, q.coef, q.t0, 0.5 * q.h, 2 * q.points + 1, q.sigma, q.seed,
-                             q.U);
+            k::g::add_sample(phases_[kSmp], m, q.coef, q.t0, 0.5 * q.h, 2 * q.points + 1, q.sigma,
+                             q.seed, q.U);
         k::g::add_cast(in, (long)nn, q.A, A);
         k::g::add_cast(in, (long)nm, q.B, B);
         if (abm)
@@ -281,11 +281,18 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         for (auto& kv : phases_[pi]) {
             const k::g::Launch& Lr = kv.second;
             if (Lr.nsys() <= 0 || Lr.blocks() == 0) continue;
-            // streams: the map on the main stream; the powers, level tables and
-            // ABM start-up on the side stream, beside the leaves and the first
-            // chunk pass; the rest after both
+            // streams: the FP64 copies and the map on the main stream; the
+            // powers, level tables and ABM start-up on the side stream, beside
+            // the input samples, the leaves and the first chunk pass; the rest
+            // after both (roles: 0 copies and map, 5 samples, 1 powers and
+            // tables, 4 start-up, 2 leaves and chunks, 3 the rest)
             const int pos = (int)pi;
-            const int role = pos <= kPost ? 0 : pos <= kStart ? 1 : pos <= kChunk1 ? 2 : 3;
+            const int role = pos == kSmp     ? 5
+                             : pos <= kPost  ? 0
+                             : pos < kStart  ? 1
+                             : pos == kStart ? 4
+                             : pos <= kChunk1 ? 2
+                                              : 3;
             Ref r{&Lr, align(off), 0, stage_of(pos), role, pos};
             r.start = align(r.args + Lr.args.size());
             off = r.start + Lr.start.size() * sizeof(unsigned);
@@ -382,10 +389,19 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
     if (side && !marks) {
         if (!side->stream || !side->fork || !side->join)
             throw std::invalid_argument("BatchSolve::run: incomplete side stream");
+        // the power tables start right after the map; the start-up reads the
+        // input samples, so without an event for them they come first
         queue(s, 0, fan);
+        if (!side->inputs) queue(s, 5, fan);
         PARODE_HIP(hipEventRecord(side->fork, s));
         PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
         queue(side->stream, 1, nullptr);
+        if (side->inputs) {
+            queue(s, 5, fan);
+            PARODE_HIP(hipEventRecord(side->inputs, s));
+            PARODE_HIP(hipStreamWaitEvent(side->stream, side->inputs, 0));
+        }
+        queue(side->stream, 4, nullptr);
         PARODE_HIP(hipEventRecord(side->join, side->stream));
         queue(s, 2, fan);
         PARODE_HIP(hipStreamWaitEvent(s, side->join, 0));
@@ -393,7 +409,7 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
         return;
     }
     if (!marks) {
-        for (int role = 0; role < 4; ++role) queue(s, role, fan);
+        for (int role : {0, 5, 1, 4, 2, 3}) queue(s, role, fan);
         return;
     }
     int stage = 0;
@@ -432,6 +448,7 @@ struct Group {
         fan.destroy();
         if (side.fork) (void)hipEventDestroy(side.fork);
         if (side.join) (void)hipEventDestroy(side.join);
+        if (side.inputs) (void)hipEventDestroy(side.inputs);
         if (side.stream) (void)hipStreamDestroy(side.stream);
         for (hipEvent_t e : marks)
             if (e) (void)hipEventDestroy(e);
@@ -513,6 +530,7 @@ int ParODE_GroupExecute(void* group, void* stream)
             PARODE_HIP(hipStreamCreate(&g->side.stream));
             PARODE_HIP(hipEventCreateWithFlags(&g->side.fork, hipEventDisableTiming));
             PARODE_HIP(hipEventCreateWithFlags(&g->side.join, hipEventDisableTiming));
+            PARODE_HIP(hipEventCreateWithFlags(&g->side.inputs, hipEventDisableTiming));
         }
         if (!g->fan.start) g->fan.create();
         g->batch->run(static_cast<hipStream_t>(stream), g->stage, g->desc, !g->uploaded, nullptr,

@@ -232,7 +232,7 @@ void to_f64(hipStream_t s, long count, const T* in, double* out);
 // launches in key order.
 namespace g {
 enum Kind {
-    kPrep, kGemm16, kGemmKs2, kGemmT64, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
+    kPrep, kGemm, kGemmKs2, kRk4Combine, kAbmCombine, kTranspose, kPowersSmall, kTables,
     kSample, kStartup, kStartOut, kLeavesPlain, kLeavesFac, kLeavesSimt, kOutputsGemm1,
     kOutputsGemm2, kOutputsGemm4,
     kChunksFac16, kChunksFac32, kChunksFac48, kChunksFac64, kChunksFac96, kChunksFac128,

@@ -259,6 +259,13 @@ __global__ __launch_bounds__(256 * KSPLIT) void k_gemm_f64_ks(int r, int kk, int
 // the arithmetic per byte of operands of the 32 x 32 tiles.
 constexpr int kG64 = 64, kG64KB = 16;
 
+// 64 x 64 tiles once the padding of the edge tiles costs less than the 16 x 16
+// kernel's lower rate (about half)
+__host__ __device__ constexpr bool gemm_wide(int r, int kk, int c)
+{
+    return r >= 96 && c >= 96 && kk >= 64;
+}
+
 __device__ __forceinline__ void d_gemm_f64_t64(const VBlock vb, int r, int kk, int c,
                                                const double* __restrict__ X,
                                                const double* __restrict__ Y,
@@ -4172,18 +4179,17 @@ __global__ __launch_bounds__(256) void kg_prep(const Prep* __restrict__ args,
     }
 }
 
-__global__ __launch_bounds__(256) void kg_gemm16(const Gemm* __restrict__ args,
-                                                 const unsigned* __restrict__ st, int ns)
+// FP64 products of one phase in one launch: each system's 64 x 64 tiles, or,
+// for a narrow product, its K-split 16 x 16 tiles (the two tilings in separate
+// launches would run one after the other on the stream)
+__global__ __launch_bounds__(256) void kg_gemm(const Gemm* __restrict__ args,
+                                               const unsigned* __restrict__ st, int ns)
 {
     PARODE_GROUP(Gemm);
-    d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
-}
-
-__global__ __launch_bounds__(256) void kg_gemm_t64(const Gemm* __restrict__ args,
-                                                   const unsigned* __restrict__ st, int ns)
-{
-    PARODE_GROUP(Gemm);
-    d_gemm_f64_t64(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
+    if (gemm_wide(a.r, a.kk, a.c))
+        d_gemm_f64_t64(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
+    else
+        d_gemm_f64_mfma(vb, a.r, a.kk, a.c, a.X, a.Y, a.Z, a.ld);
 }
 
 __global__ __launch_bounds__(512) void kg_gemm_ks2(const Gemm* __restrict__ args,
@@ -4488,14 +4494,14 @@ void add_gemm_ld(Phase& ph, int r, int kk, int c, const double* X, int ldx, cons
 {
     if (r <= 0 || c <= 0) return;
     const GemmLd ld{ldx, ldy, ldz, add, shift};
-    // 64 x 64 tiles once the padding of the edge tiles costs less than the
-    // 16 x 16 kernel's lower rate (about half)
-    if (r >= 96 && c >= 96 && kk >= 64) {
+    // (a work-group's matrix-core instructions per wave: kk for a 64 x 64 tile,
+    // kk / 16 for a K-split 16 x 16 tile)
+    if (gemm_wide(r, kk, c)) {
         const long gx = (c + kG64 - 1) / kG64, gy = (r + kG64 - 1) / kG64;
-        append(ph, kGemmT64, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, (float)kk);
+        append(ph, kGemm, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, (float)kk);
     } else {
         const long gx = (c + 15) / 16, gy = (r + 15) / 16;
-        append(ph, kGemm16, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, (float)kk);
+        append(ph, kGemm, false, Gemm{r, kk, c, X, Y, Z, ld, gx}, gx * gy, 0, kk / 16.0f);
     }
 }
 
@@ -4779,9 +4785,8 @@ void run(hipStream_t s, const Launch& L, const void* dArgs, const unsigned* dSta
     } while (0)
     switch (L.kind) {
     case kPrep: PARODE_RUN(kg_prep, Prep, 256); break;
-    case kGemm16: PARODE_RUN(kg_gemm16, Gemm, 256); break;
+    case kGemm: PARODE_RUN(kg_gemm, Gemm, 256); break;
     case kGemmKs2: PARODE_RUN(kg_gemm_ks2, Gemm, 512); break;
-    case kGemmT64: PARODE_RUN(kg_gemm_t64, Gemm, 256); break;
     case kRk4Combine: PARODE_RUN(kg_rk4_combine, Rk4Comb, 256); break;
     case kAbmCombine: PARODE_RUN(kg_abm_combine, AbmComb, 256); break;
     case kTranspose: PARODE_RUN(kg_transpose, Transpose, 256); break;
