@@ -67,7 +67,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         sys_[i].core.reset(new ScanCore(q.method, q.n, q.m, q.p, !q.C, !q.D, s.nPad, opt, arena));
         ScanCore& c = *sys_[i].core;
         if (!c.Sks32_.get()) sys_[i].copies.alloc(2 * ks_half(s.nPad, s.na), arena);
-        s.lc = c.chunk_log(s.nScan, s.nPad);
+        s.lc = c.chunk_log(s.nScan, s.nPad, true);
         s.lt = c.top_level(s.nScan, s.nPad);
         for (int l = s.lc; l < s.lt; ++l)
             if (pairs_at(s.nScan, l) > k::kKsMaxPairs)
@@ -174,8 +174,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             if (!c.level_is_f64(l)) n32 = l + 1;
             if (c.level_is_delta(l)) nd = l + 1;
         }
+        // (levels below lc are read by no kernel here: the chunk passes take the map)
         k::g::add_tables(phases_[kTables], na, c.nap_, c.nrp_, n32, nd, PT, c.PT32_.get(),
-                         c.PTm_.get());
+                         c.PTm_.get(), sh[i].lc);
         // the ABM start-up (AdamsMoulton.cpp:29-104) and its three outputs
         float* yScan = q.Y;
         if (abm) {

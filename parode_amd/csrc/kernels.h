@@ -279,8 +279,9 @@ void add_abm_combine(Phase& ph, int n, int m, double h, const double* A, const d
                      const double* B, const double* AB, double* MT, double* Bbar);
 void add_transpose(Phase& ph, int rows, int cols, const double* in, double* out);
 bool add_powers_small(Phase& ph, int na, int L, double* PT);
+// (levels [first, nlev) only)
 void add_tables(Phase& ph, int na, int nap, int nrp, int nlev, int deltaBelow, const double* PT64,
-                float* PT32, float* PTm);
+                float* PT32, float* PTm, int first = 0);
 void add_sample(Phase& ph, int m, const double* coef, double t0, double dt, long S, double sigma,
                 uint64_t seed, double* out);
 void add_startup(Phase& ph, int n, int m, double h, const float* A, const float* B,

@@ -4539,10 +4539,18 @@ bool add_powers_small(Phase& ph, int na, int L, double* PT)
 }
 
 void add_tables(Phase& ph, int na, int nap, int nrp, int nlev, int deltaBelow, const double* PT64,
-                float* PT32, float* PTm)
+                float* PT32, float* PTm, int first)
 {
-    if (nlev <= 0) return;
+    if (first < 0) first = 0;
+    if (nlev - first <= 0) return;
     if (!PTm) nap = nrp = na;
+    // the same tables from level `first` on: every pointer moved by `first` levels
+    const long nn = (long)na * na, per = (long)nrp * nap;
+    PT64 += first * nn;
+    if (PT32) PT32 += first * nn;
+    if (PTm) PTm += first * per;
+    nlev -= first;
+    deltaBelow = deltaBelow > first ? deltaBelow - first : 0;
     const long b = blocks_for((long)nrp * nap * nlev, 256, 512);
     append(ph, kTables, false, Tables{na, nap, nrp, nlev, deltaBelow, PT64, PT32, PTm, b}, b);
 }

@@ -248,16 +248,25 @@ bool ScanCore::structured_level0(long nReal, long nPad) const
 // companion map's dense block row per step).  lc is 7, or less so that a
 // launch has at least 1024 chunks (64 work-groups); at least one tree level
 // is launched above them.
-int ScanCore::chunk_log(long nReal, long nPad) const
+int ScanCore::chunk_log(long nReal, long nPad, bool grouped) const
 {
-    constexpr int kChunkLog = 7, kMinChunksLog = 10;
+    constexpr int kChunkLog = 7, kMinChunksLog = 10, kMinGroupedChunksLog = 6;
     if (xIsDouble() || (level_kernel() & 9) != 0) return 0;
     // ABM: n <= 64 with the reference's float map, n <= 128 factored (FP64 map)
     const bool abmOk = opt_.precision == kPlain32 ? k::abm_chunks_supported(n_)
                                                   : k::abm_chunks_factored_supported(n_);
     if (method_ == kABM ? !abmOk : !k::chunks_dense_supported(na_)) return 0;
-    const int lc = std::min(kChunkLog, ilog2(nPad) - kMinChunksLog);
-    if (lc < 2 || top_level(nReal, nPad) <= lc) return 0;
+    // alone, a system needs 2^10 chunks to fill the GPU; in a grouped launch the
+    // other systems fill it, so its chunks run up to 2^7 steps (a work-group
+    // loads the whole map into registers once for its chunks) below the first
+    // tree level that the top or an FP64 table takes
+    const int lt = top_level(nReal, nPad);
+    int lc = std::min(kChunkLog, ilog2(nPad) - (grouped ? kMinGroupedChunksLog : kMinChunksLog));
+    if (grouped) {
+        lc = std::min(lc, lt - 1);
+        while (lc > 0 && level_is_f64(lc - 1)) --lc;
+    }
+    if (lc < 2 || lt <= lc) return 0;
     // (the chunk kernels read the FP64 map, so the FP32 tables' form, plain
     // or delta, does not matter below lc)
     for (int l = 0; l < lc; ++l)

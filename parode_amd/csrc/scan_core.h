@@ -118,7 +118,8 @@ public:
     bool structured_level1(long nReal, long nPad) const;
     // log2 of the chunk length when the lowest levels run as chunk
     // recurrences (0: they run as tree levels)
-    int chunk_log(long nReal, long nPad) const;
+    // grouped: the rule for a system in a grouped launch (longer chunks)
+    int chunk_log(long nReal, long nPad, bool grouped = false) const;
 
     // Autotuner: times every kernel that can run each level of this shape
     // (K-split tiles, narrow or wide matrix-core tiles, SIMT) on this core's
