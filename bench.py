@@ -198,10 +198,9 @@ def served_leg(args, torch, api, systems, dev):
 
     def one_pass():
         tk = srv.submit_device_batch(items)
-        for t in tk:
-            st, l, q = srv.wait(t)
-            assert st == 0, st
-            lat.append(l)
+        st, l, q = srv.wait_batch(tk)
+        assert not st.any(), st
+        lat.extend(l.tolist())
 
     for _ in range(args.warmup):
         one_pass()

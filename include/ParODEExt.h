@@ -207,6 +207,13 @@ int ParODE_ServerSubmitDeviceBatch(void* server, int count, const int* solver,
 /* Blocks until the ticket completes; returns its status.  timesMs (may be
  * NULL) receives [submission -> completion, submission -> admission]. */
 int ParODE_ServerWait(void* server, long ticket, double* timesMs);
+/* ParODE_ServerWait over count tickets, in order: status[i] receives ticket
+ * i's status and timesMs[2 i], timesMs[2 i + 1] (timesMs may be NULL) its two
+ * times.  Returns ParODE_OK once all have completed (each status says how), or
+ * ParODE_InvalidSystem for bad arguments.  No counterpart in the reference;
+ * for callers that hand a batch over with ParODE_ServerSubmitDeviceBatch. */
+int ParODE_ServerWaitBatch(void* server, int count, const long* tickets, int* status,
+						   double* timesMs);
 int ParODE_ServerDrain(void* server);
 /* stats[8]: submitted, completed, failed, running, queued, peak running,
  * arena bytes, peak arena bytes in use. */

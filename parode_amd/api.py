@@ -24,6 +24,7 @@ EXT_SYMBOLS = ["ParODE_SetLevelKernel", "ParODE_SetPrecision", "ParODE_GetPrecis
                "ParODE_PlanCapture", "ParODE_PlanReplay", "ParODE_PlanAutotune",
                "ParODE_ScheduleClear", "ParODE_PlanChunkLog",
                "ParODE_ServerCreate", "ParODE_ServerSubmit", "ParODE_ServerWait",
+               "ParODE_ServerWaitBatch",
                "ParODE_ServerDrain", "ParODE_ServerStats", "ParODE_ServerFootprint",
                "ParODE_ServerDestroy", "ParODE_SolveNoAlloc", "ParODE_NoAllocScratchBytes",
                "ParODE_ServerSubmitDevice", "ParODE_ServerGroupStats", "ParODE_GroupCreate",
@@ -156,6 +157,9 @@ def lib():
         L.ParODE_GroupDestroy.restype = _i32
         L.ParODE_ServerWait.argtypes = [_vp, _l, _dp]
         L.ParODE_ServerWait.restype = _i32
+        L.ParODE_ServerWaitBatch.argtypes = [_vp, _i32, ctypes.POINTER(_l),
+                                             ctypes.POINTER(_i32), _dp]
+        L.ParODE_ServerWaitBatch.restype = _i32
         L.ParODE_ServerDrain.argtypes = [_vp]
         L.ParODE_ServerDrain.restype = _i32
         L.ParODE_ServerStats.argtypes = [_vp, ctypes.POINTER(_l)]
@@ -522,6 +526,22 @@ class Server:
         st = lib().ParODE_ServerWait(self.handle, ticket, ms)
         self._keep.pop(ticket, None)
         return st, ms[0], ms[1]
+
+    def wait_batch(self, tickets):
+        """Waits for every ticket (in order) in one call; returns the statuses,
+        the submission-to-completion and the submission-to-admission times (ms)
+        as arrays."""
+        k = len(tickets)
+        tk = (ctypes.c_long * k)(*tickets)
+        st = (ctypes.c_int * k)()
+        ms = np.zeros(2 * k)
+        r = lib().ParODE_ServerWaitBatch(self.handle, k, tk, st,
+                                         ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if r != 0:
+            raise ParODEError(r, "ParODE_ServerWaitBatch")
+        for t in tickets:
+            self._keep.pop(t, None)
+        return np.frombuffer(st, np.int32).copy(), ms[0::2].copy(), ms[1::2].copy()
 
     def drain(self):
         lib().ParODE_ServerDrain(self.handle)

@@ -1,6 +1,8 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <exception>
+#include <thread>
 #include <cstring>
 #include <stdexcept>
 
@@ -94,7 +96,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         if (ph == kChunk2) return kStageChunks2;
         return kStageOutputs;
     };
-    for (size_t i = 0; i < systems.size(); ++i) {
+    // one system's launches into the phase set P (systems are planned on a few
+    // threads, each into a phase set of its own, merged in system order)
+    auto plan = [&](size_t i, std::vector<k::g::Phase>& P) {
         const BatchSystem& q = systems[i];
         const Shape& s = sh[i];
         ScanCore& c = *sys_[i].core;
@@ -114,9 +118,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         const int pp = pad8(p);
         // inputs, FP64 copies, output tables (RungeKutta4thOrder.cpp:36-70,
         // AdamsMoulton.cpp:238-316 take A and B; SystemOutput.cl:13-60 C and D)
-        k::g::Phase& in = phases_[kIn];
+        k::g::Phase& in = P[kIn];
         if (q.coef)
-            k::g::add_sample(phases_[kSmp], m, q.coef, q.t0, 0.5 * q.h, 2 * q.points + 1, q.sigma,
+            k::g::add_sample(P[kSmp], m, q.coef, q.t0, 0.5 * q.h, 2 * q.points + 1, q.sigma,
                              q.seed, q.U);
         k::g::add_cast(in, (long)nn, q.A, A);
         k::g::add_cast(in, (long)nm, q.B, B);
@@ -127,32 +131,32 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         if (q.C) k::g::add_transpose_pad(in, n, p, pp, q.C, c.CT64_.get());
         if (q.D) k::g::add_transpose_pad(in, m, p, pp, q.D, c.DT64_.get());
         // products of A, in the per-system order
-        k::g::add_gemm(phases_[kR1], n, n, n, A, A, A2);
-        k::g::add_gemm(phases_[kR1], n, n, m, A, B, AB);
+        k::g::add_gemm(P[kR1], n, n, n, A, A, A2);
+        k::g::add_gemm(P[kR1], n, n, m, A, B, AB);
         if (!abm) {
-            k::g::add_gemm(phases_[kR2], n, n, n, A2, A, A3);
-            k::g::add_gemm(phases_[kR2], n, n, m, A2, B, A2B);
-            k::g::add_gemm(phases_[kR3], n, n, n, A, A3, A4);
-            k::g::add_gemm(phases_[kR3], n, n, m, A3, B, A3B);
-            k::g::add_rk4_combine(phases_[kComb], n, m, q.h, A, A2, A3, A4, B, AB, A2B, A3B, PT,
+            k::g::add_gemm(P[kR2], n, n, n, A2, A, A3);
+            k::g::add_gemm(P[kR2], n, n, m, A2, B, A2B);
+            k::g::add_gemm(P[kR3], n, n, n, A, A3, A4);
+            k::g::add_gemm(P[kR3], n, n, m, A3, B, A3B);
+            k::g::add_rk4_combine(P[kComb], n, m, q.h, A, A2, A3, A4, B, AB, A2B, A3B, PT,
                                   c.Bbar64_.get());
-            k::g::add_transpose(phases_[kPost], n, 3 * m, c.Bbar64_.get(), c.BbT64_.get());
+            k::g::add_transpose(P[kPost], n, 3 * m, c.Bbar64_.get(), c.BbT64_.get());
         } else {
-            k::g::add_abm_combine(phases_[kComb], n, m, q.h, A, A2, B, AB, PT, c.Bbar64_.get());
-            k::g::add_transpose(phases_[kPost], n, m, B, c.BfT64_.get());
-            k::g::add_transpose(phases_[kPost], n, m, AB, c.BfT64_.get() + nm);
+            k::g::add_abm_combine(P[kComb], n, m, q.h, A, A2, B, AB, PT, c.Bbar64_.get());
+            k::g::add_transpose(P[kPost], n, m, B, c.BfT64_.get());
+            k::g::add_transpose(P[kPost], n, m, AB, c.BfT64_.get() + nm);
             // B-bar itself for the vector-ALU leaves when the factored ones do not fit
             if (!k::g::leaves_fit(n, m, 5, 2, 2, 2))
-                k::g::add_transpose(phases_[kPost], n, 5 * m, c.Bbar64_.get(), c.BbT64_.get());
+                k::g::add_transpose(P[kPost], n, 5 * m, c.Bbar64_.get(), c.BbT64_.get());
         }
         // the FP64 power table P_{l+1} = P_l^2 (transposed), up to P_{L-1}: the
         // top of the tree reads levels below L, and one batch has no carry
-        if (!k::g::add_powers_small(phases_[kPow0], na, L, PT)) {
+        if (!k::g::add_powers_small(P[kPow0], na, L, PT)) {
             for (int l = 0; l + 1 < L; ++l) {
                 const double* Q = PT + l * na2;
                 double* Q2 = PT + (l + 1) * na2;
                 if (!abm) {
-                    k::g::add_gemm(phases_[kPow0 + 4 * l], na, na, na, Q, Q, Q2);
+                    k::g::add_gemm(P[kPow0 + 4 * l], na, na, na, Q, Q, Q2);
                     continue;
                 }
                 // ABM: block row i of M^j is the last block row R_{j-3+i} of
@@ -161,10 +165,10 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
                 // b is Q_0[:, 3n:4n] times the last n rows of block column b - 1,
                 // plus that column shifted down by n rows: 56 n^3 flop instead of
                 // the 128 n^3 of the full square.
-                k::g::add_gemm_ld(phases_[kPow0 + 4 * l], na, na, n, Q, na, Q, na, Q2, na,
+                k::g::add_gemm_ld(P[kPow0 + 4 * l], na, na, n, Q, na, Q, na, Q2, na,
                                   nullptr, 0);
                 for (int b = 1; b < 4; ++b)
-                    k::g::add_gemm_ld(phases_[kPow0 + 4 * l + b], na, n, n, PT + 3 * n, na,
+                    k::g::add_gemm_ld(P[kPow0 + 4 * l + b], na, n, n, PT + 3 * n, na,
                                       Q2 + 3L * n * na + (b - 1) * n, na, Q2 + b * n, na,
                                       Q2 + (b - 1) * n, n);
             }
@@ -175,13 +179,13 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             if (c.level_is_delta(l)) nd = l + 1;
         }
         // (levels below lc are read by no kernel here: the chunk passes take the map)
-        k::g::add_tables(phases_[kTables], na, c.nap_, c.nrp_, n32, nd, PT, c.PT32_.get(),
+        k::g::add_tables(P[kTables], na, c.nap_, c.nrp_, n32, nd, PT, c.PT32_.get(),
                          c.PTm_.get(), sh[i].lc);
         // the ABM start-up (AdamsMoulton.cpp:29-104) and its three outputs
         float* yScan = q.Y;
         if (abm) {
-            k::g::add_startup(phases_[kStart], n, m, q.h, q.A, q.B, q.x0, q.U, c.seed());
-            k::g::add_start_outputs(phases_[kOut], n, m, p, pp, q.C != nullptr,
+            k::g::add_startup(P[kStart], n, m, q.h, q.A, q.B, q.x0, q.U, c.seed());
+            k::g::add_start_outputs(P[kOut], n, m, p, pp, q.C != nullptr,
                                     q.D != nullptr, c.CT64_.get(), c.DT64_.get(), c.seed(), q.U,
                                     q.Y);
             yScan = q.Y + 3L * p;
@@ -202,22 +206,22 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             k::LeafComb G;
             k::abm_leaf_factors(q.h, G);
             ok = k::g::leaves_fit(n, m, 5, 2, 2, 2)
-                     ? k::g::add_leaves(phases_[kLeaves], n, m, 5, 2, na, 3 * n, s.nScan, nLeaf,
+                     ? k::g::add_leaves(P[kLeaves], n, m, 5, 2, na, 3 * n, s.nScan, nLeaf,
                                         c.BfT64_.get(), G, q.U, 2, 0, 2, X, s.lc == 0)
-                     : k::g::add_leaves_simt(phases_[kLeaves], n, m, 5, na, 3 * n, s.nScan,
+                     : k::g::add_leaves_simt(P[kLeaves], n, m, 5, na, 3 * n, s.nScan,
                                              nLeaf, c.BbT64_.get(), q.U, 2, 0, 2, X, s.lc == 0);
         } else {
             ok = k::g::leaves_fit(n, m, 3, 0, 2, 1)
-                     ? k::g::add_leaves(phases_[kLeaves], n, m, 3, 0, na, 0, s.nScan, nLeaf,
+                     ? k::g::add_leaves(P[kLeaves], n, m, 3, 0, na, 0, s.nScan, nLeaf,
                                         c.BbT64_.get(), k::LeafComb{}, q.U, 2, 0, 1, X, true)
-                     : k::g::add_leaves_simt(phases_[kLeaves], n, m, 3, na, 0, s.nScan, nLeaf,
+                     : k::g::add_leaves_simt(P[kLeaves], n, m, 3, na, 0, s.nScan, nLeaf,
                                              c.BbT64_.get(), q.U, 2, 0, 1, X, true);
         }
         if (!ok) throw std::invalid_argument("BatchSolve: leaves do not fit the kernel");
         // chunk recurrences (levels below lc)
         if (s.lc > 0) {
             for (int pass = 0; pass < 2; ++pass) {
-                k::g::Phase& ph = phases_[pass ? kChunk2 : kChunk1];
+                k::g::Phase& ph = P[pass ? kChunk2 : kChunk1];
                 ok = abm ? k::g::add_abm_chunks(ph, pass == 1, n, s.lc, s.nScan, s.nPad, q.h, A,
                                                 A2, X)
                          : k::g::add_dense_chunks(ph, pass == 1, na, s.lc, s.nScan, s.nPad, PT,
@@ -236,7 +240,7 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             bool f64;
             int ldp;
             const void* T = table(l, f64, ldp);
-            k::g::add_level(phases_[kUp0 + upSteps - (s.lt - l)], false, f64,
+            k::g::add_level(P[kUp0 + upSteps - (s.lt - l)], false, f64,
                             !f64 && c.level_is_delta(l), na, ldp, 1L << l, pairs_at(s.nScan, l),
                             T, nullptr, X, nullptr, 0);
         }
@@ -244,16 +248,16 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         if (s.lt < L) {
             int l64 = L;
             while (l64 > 0 && c.level_is_f64(l64 - 1)) --l64;
-            k::g::add_top(phases_[kTop], na, s.lt, L, l64, s.nScan, c.PT32_.get(), PT, X,
+            k::g::add_top(P[kTop], na, s.lt, L, l64, s.nScan, c.PT32_.get(), PT, X,
                           c.seed(), c.tot());
         } else {
-            k::g::add_root(phases_[kTop], na, s.nPad - 1, c.seed(), X, c.tot());
+            k::g::add_root(P[kTop], na, s.nPad - 1, c.seed(), X, c.tot());
         }
         // down levels lt-1 .. lc: each writes the next one's copy of X[R]
         float* Sk = c.Sks32_.get() ? c.Sks32_.get() : sys_[i].copies.get();
         const size_t half = c.Sks32_.get() ? c.ksHalf_ : (size_t)ks_half(s.nPad, na);
         if (s.lt - 1 >= s.lc)
-            k::g::add_gather(phases_[kGather], na, 1L << (s.lt - 1), pairs_at(s.nScan, s.lt - 1),
+            k::g::add_gather(P[kGather], na, 1L << (s.lt - 1), pairs_at(s.nScan, s.lt - 1),
                              X, Sk);
         for (int l = s.lt - 1, j = 0; l >= s.lc; --l, ++j) {
             bool f64;
@@ -261,20 +265,42 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             const void* T = table(l, f64, ldp);
             float* S = Sk + (size_t)(j & 1) * half;
             float* Sn = l - 1 >= s.lc ? Sk + (size_t)((j + 1) & 1) * half : nullptr;
-            k::g::add_level(phases_[kDown0 + j], true, f64, !f64 && c.level_is_delta(l), na, ldp,
+            k::g::add_level(P[kDown0 + j], true, f64, !f64 && c.level_is_delta(l), na, ldp,
                             1L << l, pairs_at(s.nScan, l), T, S, X, Sn,
                             Sn ? pairs_at(s.nScan, l - 1) : 0);
         }
         // outputs (SystemOutput.cl:13-60; strides RungeKutta4thOrder.cpp:209,
         // AdamsMoulton.cpp:398)
-        if (!k::g::add_outputs(phases_[kOut], n, m, p, pp, na, abm ? 3 : 0, q.C != nullptr,
+        if (!k::g::add_outputs(P[kOut], n, m, p, pp, na, abm ? 3 : 0, q.C != nullptr,
                                q.D != nullptr, s.nScan, c.CT64_.get(), c.DT64_.get(), X, q.U, 2,
                                abm ? 6 : 0, yScan))
             throw std::invalid_argument("BatchSolve: outputs do not fit the kernel");
+    };
+    const int parts = (int)std::min<size_t>(8, std::max<size_t>(1, systems.size() / 64));
+    if (parts == 1) {
+        for (size_t i = 0; i < systems.size(); ++i) plan(i, phases_);
+    } else {
+        std::vector<std::vector<k::g::Phase>> part(parts, std::vector<k::g::Phase>(phases_.size()));
+        std::vector<std::exception_ptr> err(parts);
+        std::vector<std::thread> th;
+        const size_t per = (systems.size() + parts - 1) / parts;
+        for (int t = 0; t < parts; ++t)
+            th.emplace_back([&, t] {
+                try {
+                    for (size_t i = t * per; i < std::min(systems.size(), (t + 1) * per); ++i)
+                        plan(i, part[t]);
+                } catch (...) {
+                    err[t] = std::current_exception();
+                }
+            });
+        for (auto& x : th) x.join();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
+        for (int t = 0; t < parts; ++t)
+            for (size_t pi = 0; pi < phases_.size(); ++pi)
+                for (auto& kv : part[t][pi]) k::g::merge_launch(phases_[pi][kv.first], std::move(kv.second));
     }
-    // the longest-running work-groups of every launch first
-    for (auto& ph : phases_)
-        for (auto& kv : ph) k::g::order_by_cost(kv.second);
+
     // pack every launch's records and block table
     auto align = [](size_t v) { return (v + 15) / 16 * 16; };
     size_t off = 0;
@@ -300,10 +326,10 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             refs_.push_back(r);
         }
     blob_.assign(align(off), 0);
-    for (const Ref& r : refs_) {
-        std::memcpy(blob_.data() + r.args, r.L->args.data(), r.L->args.size());
-        std::memcpy(blob_.data() + r.start, r.L->start.data(), r.L->start.size() * sizeof(unsigned));
-    }
+    // each launch's records with the longest-running work-groups first
+    for (const Ref& r : refs_)
+        k::g::pack_by_cost(*r.L, blob_.data() + r.args,
+                           reinterpret_cast<unsigned*>(blob_.data() + r.start));
 }
 
 const char* BatchSolve::stage_name(int stage)

@@ -254,10 +254,13 @@ struct Launch {
     int nsys() const { return (int)start.size() - 1; }
     unsigned blocks() const { return start.back(); }
 };
-// Reorders a launch's systems by decreasing cost (stable), so that the
-// longest-running work-groups are dispatched first and the short ones fill the
-// compute units around them instead of forming the launch's tail.
-void order_by_cost(Launch& L);
+// Writes a launch's argument records (args: nsys x argSize bytes) and block
+// table (start: nsys + 1) with its systems in decreasing cost (stable), so that
+// the longest-running work-groups are dispatched first and the short ones fill
+// the compute units around them instead of forming the launch's tail.
+void pack_by_cost(const Launch& L, unsigned char* args, unsigned* start);
+// Appends from's systems after into's (the same kind, flag and LDS class).
+void merge_launch(Launch& into, Launch&& from);
 // key: (kind, flag + 2 x LDS size class)
 using Phase = std::map<std::pair<int, int>, Launch>;
 

@@ -10,6 +10,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <vector>
 #include <cstring>
 #include <type_traits>
 #include <stdexcept>
@@ -4436,31 +4437,39 @@ void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds 
 
 }  // namespace
 
-void order_by_cost(Launch& L)
+void pack_by_cost(const Launch& L, unsigned char* args, unsigned* start)
 {
     const int ns = L.nsys();
-    if (ns < 2 || (int)L.cost.size() != ns || L.args.size() != (size_t)ns * L.argSize) return;
     std::vector<int> idx(ns);
     for (int i = 0; i < ns; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return L.cost[a] > L.cost[b]; });
-    bool same = true;
-    for (int i = 0; i < ns && same; ++i) same = idx[i] == i;
-    if (same) return;
-    std::vector<unsigned char> args(L.args.size());
-    std::vector<unsigned> start(1, 0u);
-    std::vector<float> cost;
-    start.reserve(ns + 1);
-    cost.reserve(ns);
+    if ((int)L.cost.size() == ns)
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return L.cost[a] > L.cost[b]; });
+    start[0] = 0;
     for (int k = 0; k < ns; ++k) {
         const int s = idx[k];
-        std::memcpy(args.data() + (size_t)k * L.argSize, L.args.data() + (size_t)s * L.argSize,
+        std::memcpy(args + (size_t)k * L.argSize, L.args.data() + (size_t)s * L.argSize,
                     L.argSize);
-        start.push_back(start.back() + (L.start[s + 1] - L.start[s]));
-        cost.push_back(L.cost[s]);
+        start[k + 1] = start[k] + (L.start[s + 1] - L.start[s]);
     }
-    L.args.swap(args);
-    L.start.swap(start);
-    L.cost.swap(cost);
+}
+
+void merge_launch(Launch& into, Launch&& from)
+{
+    if (from.nsys() <= 0) return;
+    if (into.kind < 0) {
+        into = std::move(from);
+        return;
+    }
+    if (into.kind != from.kind || into.flag != from.flag || into.argSize != from.argSize)
+        throw std::logic_error("grouped launch: merging different launches");
+    const unsigned base = into.blocks();
+    if ((unsigned long)base + from.blocks() > 0x7fffffffUL)
+        throw std::length_error("grouped launch: too many blocks");
+    into.args.insert(into.args.end(), from.args.begin(), from.args.end());
+    for (int k = 1; k <= from.nsys(); ++k) into.start.push_back(base + from.start[k]);
+    into.cost.insert(into.cost.end(), from.cost.begin(), from.cost.end());
+    into.lds = std::max(into.lds, from.lds);
 }
 
 void add_cast(Phase& ph, long count, const float* in, double* out)

@@ -709,6 +709,22 @@ int ParODE_ServerWait(void* server, long ticket, double* timesMs)
     return st;
 }
 
+int ParODE_ServerWaitBatch(void* server, int count, const long* tickets, int* status,
+                           double* timesMs)
+{
+    if (!server || count < 0 || (count > 0 && (!tickets || !status))) return ParODE_InvalidSystem;
+    parode::Server* sv = static_cast<parode::Server*>(server);
+    for (int i = 0; i < count; ++i) {
+        double lat = 0, q = 0;
+        status[i] = sv->wait(tickets[i], &lat, &q);
+        if (timesMs) {
+            timesMs[2 * i] = lat;
+            timesMs[2 * i + 1] = q;
+        }
+    }
+    return ParODE_OK;
+}
+
 int ParODE_ServerDrain(void* server)
 {
     if (!server) return ParODE_InvalidSystem;

@@ -146,8 +146,8 @@ def test_batch_submission_equals_single_submissions():
                                             o.data_ptr(), s.ucoef)
                                            for s, o in zip(systems, many)])
         assert tickets == sorted(tickets) and len(set(tickets)) == len(systems)
-        for t in tickets:
-            assert srv.wait(t)[0] == 0
+        st, lat, que = srv.wait_batch(tickets)   # ParODE_ServerWaitBatch
+        assert not st.any() and (lat >= que).all() and (que >= 0).all()
         gs = srv.group_stats()
     finally:
         srv.close()
