@@ -3565,10 +3565,34 @@ __device__ __forceinline__ void d_outputs_gemm(const VBlock vb, int n, int m, in
         TX ra[AQ];
         double rb[BQ];
         auto load = [&](int k0) {
+            const int kf = k0 + ak;
+            const TX* src = xe + kf;
+            // FP32 states in range: AQ / 4 loads of 16 bytes where aligned, else
+            // AQ / 2 loads of 8 (a quarter or half of the load instructions)
+            const bool whole = sizeof(TX) == 4 && AQ % 4 == 0 && realA && kf + AQ <= kC;
+            const uintptr_t al = reinterpret_cast<uintptr_t>(src) & 15;
+            if (whole && al == 0) {
 #pragma unroll
-            for (int q = 0; q < AQ; ++q) {
-                const int k = k0 + ak + q;
-                ra[q] = (realA && k < kC) ? xe[k] : (TX)0;
+                for (int q = 0; q < AQ; q += 4) {
+                    const float4 v = *reinterpret_cast<const float4*>(src + q);
+                    ra[q] = (TX)v.x;
+                    ra[q + 1] = (TX)v.y;
+                    ra[q + 2] = (TX)v.z;
+                    ra[q + 3] = (TX)v.w;
+                }
+            } else if (whole && (al & 7) == 0) {
+#pragma unroll
+                for (int q = 0; q < AQ; q += 2) {
+                    const float2 v = *reinterpret_cast<const float2*>(src + q);
+                    ra[q] = (TX)v.x;
+                    ra[q + 1] = (TX)v.y;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < AQ; ++q) {
+                    const int k = kf + q;
+                    ra[q] = (realA && k < kC) ? xe[k] : (TX)0;
+                }
             }
             const int kb = k0 + bk;
 #pragma unroll
@@ -3609,10 +3633,22 @@ __device__ __forceinline__ void d_outputs_gemm(const VBlock vb, int n, int m, in
         const int bk = tid >> 4, bj = (tid & 15) * BQU;
         double ra[4], rb[BQU];
         auto load = [&](int k0) {
+            const TU* src = ue + k0 + ak;
+            if (sizeof(TU) == 8 && realA && k0 + ak + 4 <= kU &&
+                (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+                // four FP64 inputs in range: two loads of 16 bytes
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int k = k0 + ak + q;
-                ra[q] = (realA && k < kU) ? (double)ue[k] : 0.0;
+                for (int q = 0; q < 4; q += 2) {
+                    const double2 v = *reinterpret_cast<const double2*>(src + q);
+                    ra[q] = (double)v.x;
+                    ra[q + 1] = (double)v.y;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = k0 + ak + q;
+                    ra[q] = (realA && k < kU) ? (double)ue[k] : 0.0;
+                }
             }
             const int kb = k0 + bk;
 #pragma unroll
