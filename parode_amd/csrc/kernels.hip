@@ -3191,29 +3191,19 @@ __device__ __forceinline__ void d_outputs_lanes(const VBlock vb, int n, int m, i
     TX* xs = reinterpret_cast<TX*>(outl_smem);   // [kOutLanes][n + 1]
     const int xp = n + 1;
     const long stride = (long)vb.gx * kOutLanes;
-    // With n <= kOutLanes, the block's cnt x n states, taken in order (element
-    // e, state c) = divmod(f, n), are spread over the threads: thread t holds
-    // f = t + kOutLanes u for u < n, so one load instruction covers kOutLanes
-    // consecutive states (one element's row after another).  The next block's
-    // states are loaded into registers while this one computes.  (FP32 states
-    // only: FP64 ones would take twice the registers)
+    // With n <= kOutLanes, thread c holds state c of every element of a block:
+    // the next block's states are loaded into registers while this one computes.
+    // (FP32 states only: FP64 ones would take twice the registers)
     constexpr int XV = sizeof(TX) == 4 ? kOutLanes : 16;
     const bool ahead = useC && n <= kOutLanes && sizeof(TX) == 4;
     TX xv[XV];
-    const int dq = n > 0 ? kOutLanes / n : 0, dr = kOutLanes - dq * n;   // kOutLanes = dq n + dr
     auto load_x = [&](long e0) {
         const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
-        int e = (int)threadIdx.x / n, c = (int)threadIdx.x - e * n;
+        const int c = threadIdx.x < n ? (int)threadIdx.x : n - 1;
 #pragma unroll
         for (int u = 0; u < XV; ++u) {
-            if (u >= n) break;
-            xv[u] = X[(e0 + (e < cnt ? e : cnt - 1)) * na + (long)oX * n + c];
-            c += dr;
-            e += dq;
-            if (c >= n) {
-                c -= n;
-                ++e;
-            }
+            const int q = u < cnt ? u : cnt - 1;
+            xv[u] = X[(e0 + q) * na + (long)oX * n + c];
         }
     };
     if (ahead && (long)vb.x * kOutLanes < N) load_x((long)vb.x * kOutLanes);
@@ -3221,18 +3211,10 @@ __device__ __forceinline__ void d_outputs_lanes(const VBlock vb, int n, int m, i
         const int cnt = (N - e0) < kOutLanes ? (int)(N - e0) : kOutLanes;
         __syncthreads();
         if (ahead) {
-            int e = (int)threadIdx.x / n, c = (int)threadIdx.x - e * n;
+            if ((int)threadIdx.x < n)
 #pragma unroll
-            for (int u = 0; u < XV; ++u) {
-                if (u >= n) break;
-                if (e < cnt) xs[e * xp + c] = xv[u];
-                c += dr;
-                e += dq;
-                if (c >= n) {
-                    c -= n;
-                    ++e;
-                }
-            }
+                for (int u = 0; u < XV; ++u)
+                    if (u < cnt) xs[u * xp + threadIdx.x] = xv[u];
         } else if (useC) {
             // sixteen elements' rows in flight per thread
             for (int c = threadIdx.x; c < n; c += kOutLanes)
