@@ -395,7 +395,10 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
             // (a small group's fork and joins would cost more than its tails)
             long blocks = 0;
             for (size_t j = i; j < e; ++j) blocks += refs_[j].L->blocks();
-            const int used = (f && wide(refs_[i].stage) && blocks >= kFanMinBlocks)
+            // (the top's launches, one work-group per system, are latency-bound:
+            // spread whatever their size)
+            const bool top = refs_[i].stage == kStageTop;
+            const int used = (f && ((wide(refs_[i].stage) && blocks >= kFanMinBlocks) || top))
                                  ? std::min(count - 1, FanStreams::kCount) : 0;
             if (used > 0) {
                 PARODE_HIP(hipEventRecord(f->start, st));

@@ -4460,7 +4460,10 @@ void append(Phase& ph, int kind, bool flag, const A& a, long blocks, size_t lds 
             float cost = 0.0f)
 {
     if (blocks < 1) return;
-    Launch& L = slot<A>(ph, kind, flag, lds_class(lds));
+    // (the top of the tree runs one 1024-thread work-group per system, one per
+    // CU whatever its LDS: its systems share one launch)
+    const bool oneClass = kind == kTopMfma || kind == kTopSimt;
+    Launch& L = slot<A>(ph, kind, flag, oneClass ? 0 : lds_class(lds));
     const size_t o = L.args.size();
     L.args.resize(o + sizeof(A));
     std::memcpy(L.args.data() + o, &a, sizeof(A));
