@@ -104,6 +104,7 @@ struct Server::Group {
     std::unique_ptr<ScanCore> core;      // one request of any shape
     char* io = nullptr;                  // device: uploads, samples, host-output staging
     char* desc = nullptr;                // device: the batch's launch records
+    std::shared_ptr<Stage> descStage;    // pinned: the launch records on their way up
     hipEvent_t done = nullptr;
     void release(Arena* arena)
     {
@@ -111,6 +112,7 @@ struct Server::Group {
         done = nullptr;
         batch.reset();
         core.reset();
+        descStage.reset();
         arena->give(io);
         arena->give(desc);
         io = desc = nullptr;
@@ -262,7 +264,7 @@ bool Server::start(Group& g)
 {
     hipStream_t s = lanes_[g.lane];
     // device io block: [uploads][samples made on the device][host outputs];
-    // pinned stage: [uploads][host outputs][launch records]
+    // pinned stage: [uploads][host outputs]; the launch records: a pinned block of their own
     size_t upTot = 0, uTot = 0, yTot = 0;
     std::vector<size_t> upOff(g.jobs.size()), uOff(g.jobs.size()), yOff(g.jobs.size());
     for (size_t i = 0; i < g.jobs.size(); ++i) {
@@ -290,6 +292,51 @@ bool Server::start(Group& g)
         const Job& j = *g.jobs[i];
         return j.r.yDevice ? j.r.Y : reinterpret_cast<float*>(dYs + yOff[i]);
     };
+    // pinned stage: [uploads][host outputs] (the launch records get a block of
+    // their own once planned); the uploads go up while the launches are planned
+    auto stage = std::make_shared<Stage>();
+    stage->bytes = upTot + yTot;
+    if (stage->bytes) {
+        stage->p = static_cast<char*>(pinned_->take(stage->bytes));
+        if (!stage->p) {
+            g.release(arena_.get());
+            return false;
+        }
+        stage->arena = pinned_.get();
+    }
+    // every host copy and every transfer stays inside its own block (checked
+    // for all requests before any copy); a large group's copies into the
+    // pinned stage are split among a few threads
+    for (size_t i = 0; i < g.jobs.size(); ++i) {
+        Job& j = *g.jobs[i];
+        const ServeRequest& r = j.r;
+        check_range("staged matrices", upOff[i], j.mats.size() * 4, upTot);
+        check_range("staged coefficients", upOff[i] + j.L.coef, j.coef.size() * 8, upTot);
+        if (r.samples && r.m)
+            check_range("staged samples", upOff[i] + j.L.U, (size_t)(2 * r.points + 1) * r.m * 8,
+                        upTot);
+        if (!r.yDevice) {
+            check_range("device outputs", upTot + uTot + yOff[i], j.L.yBytes, ioBytes);
+            check_range("staged outputs", upTot + yOff[i], j.L.yBytes, upTot + yTot);
+        }
+        j.stage = stage;
+        j.stY = upTot + yOff[i];
+    }
+    const size_t count = g.jobs.size(), parts = std::min<size_t>(8, count / 64 + 1);
+    auto copy = [&](size_t p) {
+        for (size_t i = p; i < count; i += parts) {
+            const Job& j = *g.jobs[i];
+            const ServeRequest& r = j.r;
+            std::memcpy(stage->p + upOff[i], j.mats.data(), j.mats.size() * 4);
+            if (!j.coef.empty())
+                std::memcpy(stage->p + upOff[i] + j.L.coef, j.coef.data(), j.coef.size() * 8);
+            if (r.samples && r.m)
+                std::memcpy(stage->p + upOff[i] + j.L.U, r.samples,
+                            (size_t)(2 * r.points + 1) * r.m * 8);
+        }
+    };
+    run_parts(parts, copy);
+    if (upTot) PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
     size_t descBytes = 0;
     try {
         if (g.grouped) {
@@ -330,55 +377,24 @@ bool Server::start(Group& g)
             g.core.reset(new ScanCore(j.r.solver, j.r.n, j.r.m, j.r.p, j.r.zC, j.r.zD, nScan, opt,
                                       arena_.get()));
         }
+        if (descBytes) {
+            g.descStage = std::make_shared<Stage>();
+            g.descStage->bytes = Arena::round(descBytes);
+            g.descStage->p = static_cast<char*>(pinned_->take(g.descStage->bytes));
+            if (!g.descStage->p) throw std::bad_alloc();
+            g.descStage->arena = pinned_.get();
+            check_range("launch records", 0, descBytes, g.descStage->bytes);
+        }
     } catch (std::bad_alloc&) {
+        // the upload reads the stage and writes io: it ends before they return
+        (void)hipStreamSynchronize(s);
+        for (auto& j : g.jobs) j->stage.reset();
         g.release(arena_.get());
         return false;
     }
-    auto stage = std::make_shared<Stage>();
-    stage->bytes = upTot + yTot + Arena::round(descBytes);
-    stage->p = static_cast<char*>(pinned_->take(stage->bytes));
-    if (!stage->p) {
-        g.release(arena_.get());
-        return false;
-    }
-    stage->arena = pinned_.get();
-    // every host copy and every transfer stays inside its own block (checked
-    // for all requests before any copy); a large group's copies into the
-    // pinned stage are split among a few threads
-    for (size_t i = 0; i < g.jobs.size(); ++i) {
-        Job& j = *g.jobs[i];
-        const ServeRequest& r = j.r;
-        check_range("staged matrices", upOff[i], j.mats.size() * 4, upTot);
-        check_range("staged coefficients", upOff[i] + j.L.coef, j.coef.size() * 8, upTot);
-        if (r.samples && r.m)
-            check_range("staged samples", upOff[i] + j.L.U, (size_t)(2 * r.points + 1) * r.m * 8,
-                        upTot);
-        if (!r.yDevice) {
-            check_range("device outputs", upTot + uTot + yOff[i], j.L.yBytes, ioBytes);
-            check_range("staged outputs", upTot + yOff[i], j.L.yBytes, upTot + yTot);
-        }
-        j.stage = stage;
-        j.stY = upTot + yOff[i];
-    }
-    const size_t count = g.jobs.size(), parts = std::min<size_t>(8, count / 64 + 1);
-    auto copy = [&](size_t p) {
-        for (size_t i = p; i < count; i += parts) {
-            const Job& j = *g.jobs[i];
-            const ServeRequest& r = j.r;
-            std::memcpy(stage->p + upOff[i], j.mats.data(), j.mats.size() * 4);
-            if (!j.coef.empty())
-                std::memcpy(stage->p + upOff[i] + j.L.coef, j.coef.data(), j.coef.size() * 8);
-            if (r.samples && r.m)
-                std::memcpy(stage->p + upOff[i] + j.L.U, r.samples,
-                            (size_t)(2 * r.points + 1) * r.m * 8);
-        }
-    };
-    run_parts(parts, copy);
-    check_range("launch records", upTot + yTot, descBytes, stage->bytes);
-    PARODE_HIP(hipMemcpyAsync(g.io, stage->p, upTot, hipMemcpyHostToDevice, s));
     if (g.grouped) {
-        g.batch->run(s, stage->p + upTot + yTot, g.desc, true, nullptr, &sides_[g.lane],
-                     &fans_[g.lane]);
+        g.batch->run(s, g.descStage ? g.descStage->p : nullptr, g.desc, true, nullptr,
+                     &sides_[g.lane], &fans_[g.lane]);
     } else {
         const Job& j = *g.jobs[0];
         const ServeRequest& r = j.r;
