@@ -56,7 +56,6 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
     };
     std::vector<Shape> sh(systems.size());
     sys_.resize(systems.size());
-    std::vector<size_t> at(systems.size() + 1, 0);   // each system's range of the block
     for (size_t i = 0; i < systems.size(); ++i) {
         const BatchSystem& q = systems[i];
         if (!eligible(q.method, q.n, q.m, q.p, !q.C, !q.D, q.points) || !(q.h > 0.0) || !q.A ||
@@ -67,61 +66,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         s.nPad = next_pow2(s.nScan);
         s.L = ilog2(s.nPad);
         s.na = q.method == kABM ? 4 * q.n : q.n;
-        at[i + 1] = at[i] + Arena::round(footprint(q.method, q.n, q.m, q.p, q.points));
-    }
-    // every system's buffers from its own range of one block, made on a few
-    // threads; a system whose range falls short (the footprint is an upper
-    // bound, so it should not) is made in the arena itself afterwards
-    block_.parent = arena;
-    block_.p = at.back() ? arena->take(at.back()) : nullptr;
-    if (at.back() && !block_.p) throw std::bad_alloc();
-    subs_.resize(systems.size());
-    std::vector<char> fellShort(systems.size(), 0);
-    auto make = [&](size_t i, Arena* a) {
-        const BatchSystem& q = systems[i];
-        const Shape& s = sh[i];
-        sys_[i].core.reset(new ScanCore(q.method, q.n, q.m, q.p, !q.C, !q.D, s.nPad, opt, a));
-        if (!sys_[i].core->Sks32_.get()) sys_[i].copies.alloc(2 * ks_half(s.nPad, s.na), a);
-    };
-    auto make_range = [&](size_t from, size_t to) {
-        for (size_t i = from; i < to; ++i) {
-            subs_[i].reset(new Arena(static_cast<char*>(block_.p) + at[i], at[i + 1] - at[i]));
-            try {
-                make(i, subs_[i].get());
-            } catch (std::bad_alloc&) {
-                sys_[i] = Planned{};
-                fellShort[i] = 1;
-            }
-        }
-    };
-    {
-        const int parts = (int)std::min<size_t>(8, std::max<size_t>(1, systems.size() / 64));
-        std::vector<std::exception_ptr> err(parts);
-        std::vector<std::thread> th;
-        const size_t per = (systems.size() + parts - 1) / parts;
-        for (int t = 1; t < parts; ++t)
-            th.emplace_back([&, t] {
-                try {
-                    make_range(t * per, std::min(systems.size(), (t + 1) * per));
-                } catch (...) {
-                    err[t] = std::current_exception();
-                }
-            });
-        try {
-            make_range(0, std::min(systems.size(), per));
-        } catch (...) {
-            err[0] = std::current_exception();
-        }
-        for (auto& x : th) x.join();
-        for (auto& e : err)
-            if (e) std::rethrow_exception(e);
-        for (size_t i = 0; i < systems.size(); ++i)
-            if (fellShort[i]) make(i, arena);
-    }
-    for (size_t i = 0; i < systems.size(); ++i) {
-        const BatchSystem& q = systems[i];
-        Shape& s = sh[i];
+        sys_[i].core.reset(new ScanCore(q.method, q.n, q.m, q.p, !q.C, !q.D, s.nPad, opt, arena));
         ScanCore& c = *sys_[i].core;
+        if (!c.Sks32_.get()) sys_[i].copies.alloc(2 * ks_half(s.nPad, s.na), arena);
         s.lc = c.chunk_log(s.nScan, s.nPad, true);
         s.lt = c.top_level(s.nScan, s.nPad);
         for (int l = s.lc; l < s.lt; ++l)

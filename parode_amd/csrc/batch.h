@@ -89,21 +89,6 @@ public:
     static const char* stage_name(int stage);
 
 private:
-    // The systems' buffers: one block of the caller's arena, carved by a
-    // borrowed arena per system (so that they are made on several threads).
-    // Declared before sys_, so that the buffers go back to their arenas, and
-    // the block to the caller's, after sys_ is gone (also when the
-    // constructor throws).
-    struct Block {
-        Arena* parent = nullptr;
-        void* p = nullptr;
-        ~Block()
-        {
-            if (p) parent->give(p);
-        }
-    };
-    Block block_;
-    std::vector<std::unique_ptr<Arena>> subs_;
     struct Planned {
         std::unique_ptr<ScanCore> core;
         DevArray<float> copies;      // K-split copy halves when the core has none
