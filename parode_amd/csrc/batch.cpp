@@ -309,10 +309,10 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
             const k::g::Launch& Lr = kv.second;
             if (Lr.nsys() <= 0 || Lr.blocks() == 0) continue;
             // streams: the FP64 copies and the map on the main stream; the
-            // powers, level tables and ABM start-up on the side stream, beside
-            // the input samples, the leaves and the first chunk pass; the rest
-            // after both (roles: 0 copies and map, 5 samples, 1 powers and
-            // tables, 4 start-up, 2 leaves and chunks, 3 the rest)
+            // powers and level tables on the side stream, beside the input
+            // samples, the ABM start-up, the leaves and the first chunk pass;
+            // the rest after both (roles: 0 copies and map, 5 samples, 1
+            // powers and tables, 4 start-up, 2 leaves and chunks, 3 the rest)
             const int pos = (int)pi;
             const int role = pos == kSmp     ? 5
                              : pos <= kPost  ? 0
@@ -419,20 +419,16 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
     if (side && !marks) {
         if (!side->stream || !side->fork || !side->join)
             throw std::invalid_argument("BatchSolve::run: incomplete side stream");
-        // the power tables start right after the map; the start-up reads the
-        // input samples, so without an event for them they come first
+        // the power tables start right after the map; the ABM start-up (a few
+        // small launches that read the input samples) runs on this stream, which
+        // reaches the join first
         queue(s, 0, fan);
-        if (!side->inputs) queue(s, 5, fan);
         PARODE_HIP(hipEventRecord(side->fork, s));
         PARODE_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
         queue(side->stream, 1, nullptr);
-        if (side->inputs) {
-            queue(s, 5, fan);
-            PARODE_HIP(hipEventRecord(side->inputs, s));
-            PARODE_HIP(hipStreamWaitEvent(side->stream, side->inputs, 0));
-        }
-        queue(side->stream, 4, nullptr);
         PARODE_HIP(hipEventRecord(side->join, side->stream));
+        queue(s, 5, fan);
+        queue(s, 4, fan);
         queue(s, 2, fan);
         PARODE_HIP(hipStreamWaitEvent(s, side->join, 0));
         queue(s, 3, fan);
@@ -478,7 +474,6 @@ struct Group {
         fan.destroy();
         if (side.fork) (void)hipEventDestroy(side.fork);
         if (side.join) (void)hipEventDestroy(side.join);
-        if (side.inputs) (void)hipEventDestroy(side.inputs);
         if (side.stream) (void)hipStreamDestroy(side.stream);
         for (hipEvent_t e : marks)
             if (e) (void)hipEventDestroy(e);
@@ -560,7 +555,6 @@ int ParODE_GroupExecute(void* group, void* stream)
             PARODE_HIP(hipStreamCreate(&g->side.stream));
             PARODE_HIP(hipEventCreateWithFlags(&g->side.fork, hipEventDisableTiming));
             PARODE_HIP(hipEventCreateWithFlags(&g->side.join, hipEventDisableTiming));
-            PARODE_HIP(hipEventCreateWithFlags(&g->side.inputs, hipEventDisableTiming));
         }
         if (!g->fan.start) g->fan.create();
         g->batch->run(static_cast<hipStream_t>(stream), g->stage, g->desc, !g->uploaded, nullptr,

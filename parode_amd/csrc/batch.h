@@ -71,11 +71,12 @@ public:
     // until the work on s is done.  marks, when given, are kStages + 1 events
     // recorded before each stage's launches and after the last (every launch
     // then runs on s, one after the other).  Else, with side, the FP64 power
-    // tables, the level tables and the ABM start-up run on side->stream
-    // (forked after the step map, joined before the first tree level) beside
-    // the leaves and the first chunk pass; with fan, too, the launches of the
-    // inputs, leaves, chunk-pass and output phases are spread over s and the
-    // fan streams (each phase forked from s and joined back to it).
+    // tables and the level tables run on side->stream (forked after the step
+    // map, joined before the first tree level) beside the input samples, the
+    // ABM start-up, the leaves and the first chunk pass; with fan, too, the
+    // launches of the inputs, leaves, chunk-pass and output phases (and of the
+    // top of the tree) are spread over s and the fan streams (each phase
+    // forked from s and joined back to it).
     void run(hipStream_t s, void* hostStage, void* devDesc, bool upload = true,
              const hipEvent_t* marks = nullptr, const ScanCore::SideStream* side = nullptr,
              const FanStreams* fan = nullptr) const;

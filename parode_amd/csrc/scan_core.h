@@ -63,9 +63,6 @@ public:
     struct SideStream {
         hipStream_t stream = nullptr;
         hipEvent_t fork = nullptr, join = nullptr, late = nullptr;
-        // grouped solves: the main stream's input samples are made (the side
-        // stream's start-up waits for it, the power tables do not)
-        hipEvent_t inputs = nullptr;
     };
     void build_map(hipStream_t s, const float* dA, const float* dB, double h,
                    const SideStream* side = nullptr);

@@ -152,7 +152,6 @@ Server::Server(int lanes, size_t deviceBytes, size_t pinnedBytes, int precision)
         PARODE_HIP(hipStreamCreateWithFlags(&sd.stream, hipStreamNonBlocking));
         PARODE_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
         PARODE_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
-        PARODE_HIP(hipEventCreateWithFlags(&sd.inputs, hipEventDisableTiming));
         fans_[l].create();
     }
     st_.arenaBytes = arena_->capacity();
@@ -171,7 +170,6 @@ Server::~Server()
     for (auto& sd : sides_) {
         if (sd.fork) (void)hipEventDestroy(sd.fork);
         if (sd.join) (void)hipEventDestroy(sd.join);
-        if (sd.inputs) (void)hipEventDestroy(sd.inputs);
         if (sd.stream) (void)hipStreamDestroy(sd.stream);
     }
     for (auto& f : fans_) f.destroy();
