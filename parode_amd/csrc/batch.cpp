@@ -81,7 +81,11 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
               kChunk1 = kLeaves + 1, kUp0 = kChunk1 + 1, kTop = kUp0 + upSteps,
               kGather = kTop + 1, kDown0 = kGather + 1, kChunk2 = kDown0 + upSteps,
               kOut = kChunk2 + 1;
-    phases_.resize(kOut + 1);
+    // the outputs in three sets, by the stream that runs the system's second
+    // chunk pass (kinds in turn), so that each stream's outputs start when its
+    // own chunk launches end (sets fold onto fewer streams when fewer run)
+    phases_.resize(kOut + kOutSets);
+    outPhase_ = kOut;
     auto stage_of = [&](int ph) {
         if (ph <= kIn) return kStageInputs;
         if (ph <= kPost) return kStageMap;
@@ -271,7 +275,9 @@ BatchSolve::BatchSolve(const std::vector<BatchSystem>& systems, Arena* arena)
         }
         // outputs (SystemOutput.cl:13-60; strides RungeKutta4thOrder.cpp:209,
         // AdamsMoulton.cpp:398)
-        if (!k::g::add_outputs(P[kOut], n, m, p, pp, na, abm ? 3 : 0, q.C != nullptr,
+        const int ck = s.lc > 0 ? k::g::chunks_kind(abm, abm ? n : na) : -1;
+        const int set = ck >= 0 ? chunk_lane(ck) : 0;
+        if (!k::g::add_outputs(P[kOut + set], n, m, p, pp, na, abm ? 3 : 0, q.C != nullptr,
                                q.D != nullptr, s.nScan, c.CT64_.get(), c.DT64_.get(), X, q.U, 2,
                                abm ? 6 : 0, yScan))
             throw std::invalid_argument("BatchSolve: outputs do not fit the kernel");
@@ -386,6 +392,38 @@ void BatchSolve::run(hipStream_t s, void* hostStage, void* devDesc, bool upload,
         for (size_t i = 0; i < refs_.size();) {
             if (refs_[i].role != role) {
                 ++i;
+                continue;
+            }
+            if (refs_[i].stage == kStageChunks2 || refs_[i].stage == kStageOutputs) {
+                // the second chunk pass and the outputs: a stream runs its
+                // chunk kinds, then the outputs of their systems (set w of
+                // the outputs follows the kinds of lane w), joined at the end
+                size_t e = i;
+                long blocks = 0;
+                while (e < refs_.size() && (refs_[e].stage == kStageChunks2 ||
+                                            refs_[e].stage == kStageOutputs)) {
+                    blocks += refs_[e].L->blocks();
+                    ++e;
+                }
+                const int used = (f && e - i > 1 && blocks >= kFanMinBlocks)
+                                     ? FanStreams::kCount : 0;
+                if (used > 0) {
+                    PARODE_HIP(hipEventRecord(f->start, st));
+                    for (int k = 0; k < used; ++k)
+                        PARODE_HIP(hipStreamWaitEvent(f->stream[k], f->start, 0));
+                }
+                for (size_t j = i; j < e; ++j) {
+                    const Ref& r = refs_[j];
+                    const int lane = r.stage == kStageChunks2 ? chunk_lane(r.L->kind)
+                                                              : r.phase - outPhase_;
+                    const int w = lane % (used + 1);
+                    launch(w == 0 ? st : f->stream[w - 1], r);
+                }
+                for (int k = 0; k < used; ++k) {
+                    PARODE_HIP(hipEventRecord(f->done[k], f->stream[k]));
+                    PARODE_HIP(hipStreamWaitEvent(st, f->done[k], 0));
+                }
+                i = e;
                 continue;
             }
             size_t e = i + 1;

@@ -105,6 +105,10 @@ private:
         int phase;
     };
     std::vector<Ref> refs_;
+    static constexpr int kOutSets = 3;
+    int outPhase_ = 0;   // the first of the outputs' kOutSets phases
+    // the set (and fan stream) of a chunk kind's second pass
+    static int chunk_lane(int kind) { return (kind - k::g::kChunksFac16) % kOutSets; }
 };
 
 }  // namespace parode

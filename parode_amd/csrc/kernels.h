@@ -300,6 +300,9 @@ bool add_leaves(Phase& ph, int n, int m, int kq, int kg, int na, int off, long n
 bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nReal, long nPad,
                      const double* BbT, const double* U, long sL, long oL, long rs, float* X,
                      bool writeZeros);
+// The chunk kernel's kind a system's chunk passes launch (ABM: by n; dense: by
+// n_a), or -1 when none covers it.
+int chunks_kind(bool abm, int n);
 bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad, double h,
                     const double* A64, const double* A264, float* X);
 bool add_dense_chunks(Phase& ph, bool pass2, int na, int lc, long nReal, long nPad,

@@ -4700,6 +4700,15 @@ bool add_leaves_simt(Phase& ph, int n, int m, int kq, int na, int off, long nRea
     return true;
 }
 
+int chunks_kind(bool abm, int n)
+{
+    static const int kNp[] = {16, 32, 48, 64, 96, 128, 160, 192, 224, 256};
+    if (abm ? !abm_chunks_factored_supported(n) : !chunks_dense_supported(n)) return -1;
+    int t = 0;
+    while (t < 9 && kNp[t] < n) ++t;
+    return (abm ? kChunksFac16 : kDense16) + t;
+}
+
 bool add_abm_chunks(Phase& ph, bool pass2, int n, int lc, long nReal, long nPad, double h,
                     const double* A64, const double* A264, float* X)
 {
