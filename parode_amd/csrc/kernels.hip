@@ -3962,43 +3962,53 @@ __device__ inline uint64_t mix64(uint64_t& x)
     return z ^ (z >> 31);
 }
 
+// Samples come in runs of kSampleRun steps of one input, each run starting at
+// an absolute step that is a multiple of kSampleRun: one sincos at the run's
+// start, then sin g and cos g advance by the rotation through k4 dt per step
+// (a few ulps of drift over a run).  A sample's bits depend only on its
+// (step, input), not on the grid or on s0.
+constexpr int kSampleRun = 16;
+
 __device__ __forceinline__ void d_sample_family(const VBlock vb, int m, const double* __restrict__ coef, double t0, double dt,
                                 long s0, long S, double sigma, uint64_t seed,
                                 double* __restrict__ out)
 {
-    const long total = S * m;
-    long i = (long)vb.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    // (step, input) of the sample and of the grid stride: the loop advances
-    // them without a 64-bit division per sample
+    const long q0 = s0 / kSampleRun, items = ((s0 + S - 1) / kSampleRun - q0 + 1) * m;
     const long stride = (long)vb.gx * blockDim.x;
-    long sl = i / m;
-    int j = (int)(i - sl * m);
-    const long dS = stride / m;
-    const int dJ = (int)(stride - dS * m);
-    for (; i < total; i += stride) {
+    for (long w = (long)vb.x * blockDim.x + threadIdx.x; w < items; w += stride) {
+        const long qi = w / m;
+        const int j = (int)(w - qi * m);
         const double* k = coef + 6 * j;
-        const double t = t0 + (double)(s0 + sl) * dt;
-        const double g = fma(k[4], t, k[5]);
-        double sg, cg;
-        sincos(g, &sg, &cg);   // one argument reduction for both
-        double u = fma(k[3], t, fma(k[1], cg, fma(k[0], sg, k[2])));
-        if (sigma != 0.0) {
-            // Box-Muller on two counter-keyed splitmix64 draws
-            uint64_t st = seed * 0xD1B54A32D192ED03ull + (uint64_t)(s0 + sl) * 0x9E3779B97F4A7C15ull +
-                          (uint64_t)j * 0xA24BAED4963EE407ull;
-            const double u1 = ((double)(mix64(st) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
-            const double u2 = (double)(mix64(st) >> 11) * (1.0 / 9007199254740992.0);
-            u += sigma * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
-        }
-        out[i] = u;
-        j += dJ;
-        sl += dS;
-        if (j >= m) {
-            j -= m;
-            ++sl;
+        long a = (q0 + qi) * kSampleRun;
+        const long end = a + kSampleRun < s0 + S ? a + kSampleRun : s0 + S;
+        double sg, cg, sd, cd;
+        sincos(fma(k[4], t0 + (double)a * dt, k[5]), &sg, &cg);   // one argument reduction for both
+        sincos(k[4] * dt, &sd, &cd);
+        for (; a < end; ++a) {
+            if (a >= s0) {
+                const double t = t0 + (double)a * dt;
+                double u = fma(k[3], t, fma(k[1], cg, fma(k[0], sg, k[2])));
+                if (sigma != 0.0) {
+                    // Box-Muller on two counter-keyed splitmix64 draws
+                    uint64_t st = seed * 0xD1B54A32D192ED03ull + (uint64_t)a * 0x9E3779B97F4A7C15ull +
+                                  (uint64_t)j * 0xA24BAED4963EE407ull;
+                    const double u1 = ((double)(mix64(st) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+                    const double u2 = (double)(mix64(st) >> 11) * (1.0 / 9007199254740992.0);
+                    u += sigma * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+                }
+                out[(a - s0) * m + j] = u;
+            }
+            const double s1 = fma(sg, cd, cg * sd), c1 = fma(cg, cd, -(sg * sd));
+            sg = s1;
+            cg = c1;
         }
     }
+}
+
+// work items (runs of one input) of S samples per input starting at step s0
+static inline long sample_items(int m, long s0, long S)
+{
+    return ((s0 + S - 1) / kSampleRun - s0 / kSampleRun + 1) * m;
 }
 
 __global__ void k_sample_family(int m, const double* __restrict__ coef, double t0, double dt,
@@ -4011,9 +4021,8 @@ __global__ void k_sample_family(int m, const double* __restrict__ coef, double t
 void sample_family(hipStream_t s, int m, const double* coef, double t0, double dt, long s0,
                    long S, double sigma, uint64_t seed, double* out)
 {
-    const long total = S * m;
-    if (total <= 0) return;
-    hipLaunchKernelGGL(k_sample_family, dim3(blocks_for(total, 256, 8192)), dim3(256), 0, s, m,
+    if (S * m <= 0) return;
+    hipLaunchKernelGGL(k_sample_family, dim3(blocks_for(sample_items(m, s0, S), 256, 8192)), dim3(256), 0, s, m,
                        coef, t0, dt, s0, S, sigma, seed, out);
     check_launch("sample_family");
 }
@@ -4607,7 +4616,7 @@ void add_sample(Phase& ph, int m, const double* coef, double t0, double dt, long
                 uint64_t seed, double* out)
 {
     if (S * m <= 0) return;
-    const long b = blocks_for(S * m, 256, 256);
+    const long b = blocks_for(sample_items(m, 0, S), 256, 256);
     append(ph, kSample, false, Sample{m, coef, t0, dt, S, sigma, seed, out, b}, b);
 }
 
