@@ -62,6 +62,21 @@ def test_host_samples_match_device_sampling():
     srv.close()
 
 
+@pytest.mark.parametrize("solver", [0, 1])
+def test_device_sampling_holds_over_a_long_horizon(solver):
+    """Device samples come in runs of 16 steps (one sincos, then rotations);
+    over 2^16 steps (8192 runs per input, the h/2 grid reaching t = 65) the
+    outputs equal those of the host samples to 1e-7, as on short horizons."""
+    s = synth.make_system(solver, 12, 3, 2, -5.0, -0.1, 1e-3, 1 << 16, seed=45)
+    srv = api.Server(lanes=2, device_bytes=1 << 30, pinned_bytes=64 << 20)
+    t1, Y1 = srv.submit(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h, coef=s.ucoef)
+    t2, Y2 = srv.submit(s.method, s.A, s.B, s.C, s.D, s.x0, s.steps, s.h,
+                        samples=synth.samples_f64(s))
+    assert srv.wait(t1)[0] == 0 and srv.wait(t2)[0] == 0
+    assert orc.max_rel_err(Y1, Y2.astype(np.float64)) <= 1e-7
+    srv.close()
+
+
 def test_noisy_inputs_sampled_on_device_match_the_oracle_generator():
     s = synth.make_system(0, 8, 2, 2, -5.0, -0.1, 1e-3, 1000, seed=42, noise_sigma=0.1)
     srv = api.Server(lanes=1, device_bytes=1 << 30, pinned_bytes=64 << 20)
